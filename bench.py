@@ -1,0 +1,163 @@
+"""Benchmark of the block-task path on one MI355X.
+
+Workload (one step): the Game of Life script of python/life.py, 200
+generations over a 16384 x 16384 U8 raster of Bernoulli(p=0.3) cells
+(numpy PCG64, seed 1) in 1024 x 1024 blocks, evaluated through the public API
+(map_amd.map: convolve + relational/arithmetic ops -> DAG -> fusion -> one
+hiprtc-compiled kernel per fused group). The input is resident in HBM before
+timing; each step re-evaluates the 200-generation DAG from that input.
+
+Prints one JSON line: value = cell-generations per second (whole job), with
+the roofline of the dominant kernel (one generation over all blocks, 2 bytes
+of algorithmic HBM traffic per cell: 1 read + 1 write) and a CPU baseline
+(the oracle's restatement, timed on a bounded sample on the host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def life_input(side, seed):
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = np.empty((side, side), np.uint8)
+    rows = max(1, (1 << 24) // side)
+    for r in range(0, side, rows):
+        n = min(rows, side - r)
+        out[r:r + n] = rng.random((n, side)) < 0.3
+    return out
+
+
+def life_script(M, state, gens):
+    mask = [[1, 1, 1], [1, 0, 1], [1, 1, 1]]
+    for _ in range(gens):
+        nbh = M.convolve(state, mask)
+        state = (nbh == 3) + (nbh == 2) * state
+    return state
+
+
+def cpu_baseline(s0_full, bs, budget_s=12.0):
+    """The oracle's Life (C, OpenMP over all host cores) on a bounded sample."""
+    import numpy as np
+    from oracle import ref as O
+    cores = os.cpu_count() or 1
+    side = min(s0_full.shape[0], 8192)
+    sample = np.ascontiguousarray(s0_full[:side, :side])
+    t0 = time.perf_counter()
+    O.life(sample, bs, 1)
+    one = time.perf_counter() - t0
+    gens = int(max(1, min(200, budget_s / max(one, 1e-3))))
+    t0 = time.perf_counter()
+    O.life(sample, bs, gens)
+    dt = time.perf_counter() - t0
+    return {"value": side * side * gens / dt / 1e9, "unit": "Gcell-gen/s", "cores": cores, "kind": "port",
+            "sample": "Life %dx%d U8, %d generations, blocks %dx%d (top-left corner of the same input), "
+                      "oracle/oracle.c or_life with OpenMP" % (side, side, gens, bs[0], bs[1])}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--side", type=int, default=16384)
+    ap.add_argument("--block", type=int, default=1024)
+    ap.add_argument("--gens", type=int, default=200)
+    ap.add_argument("--form", choices=["batched", "plain"], default="batched")
+    ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-gens", type=int, default=8, help="generations timed once in the plain form")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from map_amd import map as M
+    M.setupDevices("", M.DEV_GPU, "")
+    bs = (args.block, args.block)
+    s0 = life_input(args.side, 1)
+    S = M.from_array(s0, bs)
+    M.set_option("form", 1 if args.form == "batched" else 0)
+    M.set_option("graph", args.graph)
+
+    out = life_script(M, S, args.gens)
+
+    def step():
+        M.release(out)
+        M.eval(out)
+
+    # inputs to HBM and kernel compile outside the timed region
+    M.eval(out)
+    for _ in range(args.warmup):
+        step()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        dev_ms.append(M.last_eval_ms())
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / args.steps
+    cells = args.side * args.side
+    value = cells * args.gens / (wall / args.steps) / 1e9
+
+    # roofline of the per-generation kernel: launches of one step = generations
+    st = M.stats()
+    launches = args.gens
+    kern_ms = min(dev_ms) / launches
+    alg_bytes = 2.0 * cells  # one U8 read + one U8 write per cell and generation
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    res = {
+        "metric": "Life cell-generations/s (fused focal+local group, 200 generations)",
+        "value": round(value, 3),
+        "unit": "Gcell-gen/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "game-of-life %dx%d u8 Bernoulli(0.3) seed 1, %d generations, blocks %dx%d"
+                               % (args.side, args.side, args.gens, bs[0], bs[1]),
+                   "form": args.form, "graph": bool(args.graph)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_ms": round(kern_ms, 5), "algorithmic_bytes_per_launch": int(alg_bytes)},
+        "device_ms_per_step": round(min(dev_ms), 3),
+        "launches_per_step": launches,
+    }
+    if args.plain_gens > 0 and args.form == "batched":
+        # the reference-shaped form: one launch per fused group per block
+        small = life_script(M, S, args.plain_gens)
+        M.set_option("graph", 0)
+        M.set_option("form", 0)
+        M.eval(small)
+        M.release(small)
+        t0 = time.perf_counter()
+        M.eval(small)
+        pw = time.perf_counter() - t0
+        M.set_option("form", 1)
+        M.release(small)
+        t0 = time.perf_counter()
+        M.eval(small)
+        bw = time.perf_counter() - t0
+        res["forms"] = {"generations": args.plain_gens,
+                        "plain_gcellgen_s": round(cells * args.plain_gens / pw / 1e9, 3),
+                        "batched_gcellgen_s": round(cells * args.plain_gens / bw / 1e9, 3),
+                        "plain_launches": args.plain_gens * (args.side // args.block) ** 2}
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(s0, bs)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

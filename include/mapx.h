@@ -1,0 +1,152 @@
+/*
+ * mapx.h - C ABI of the MI355X map-algebra backend (libmapx.so).
+ *
+ * The ma_* entry points are the drop-in boundary: the same names, argument
+ * order, by-value struct layouts and enum values as the reference's C ABI
+ * (front/bindings.hpp:21-94), which python/map.py binds with ctypes
+ * (python/map.py:723-830). Each declaration cites the reference function it
+ * replaces. Entry points marked "extension" have no reference counterpart.
+ *
+ * Differences in error behaviour: the reference asserts and aborts on every
+ * error (bindings.cpp, OclEnv.cpp:1264-1271). Here a failing call returns
+ * NULL (node factories), a non-zero int, or an empty variant (dtype 0), and
+ * ma_lastError() describes the failure; the process keeps running.
+ */
+#ifndef MAPX_H_
+#define MAPX_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef MAPX_NODE_TYPE
+#define MAPX_NODE_TYPE struct ma_node
+#endif
+typedef MAPX_NODE_TYPE ma_node_t; /* opaque DAG node, owned by the runtime */
+
+/* VariantType: 8-byte value union + dtype tag (python/map.py:91-109). */
+typedef struct {
+  union {
+    unsigned char b8;
+    signed char s8;
+    short s16;
+    int s32;
+    long long s64;
+    unsigned char u8;
+    unsigned short u16;
+    unsigned int u32;
+    unsigned long long u64;
+    float f32;
+    double f64;
+  } u;
+  int dtype;
+} ma_variant_t;
+
+/* Array4<int> with N=2: {int arr[2]; int n} (util/Array4.hpp:37-44; python/map.py:157-161). */
+typedef struct {
+  int arr[2];
+  int n;
+} ma_dim2_t;
+
+/* Enum values: python/map.py:35-83 (DataTypeEnum F32=1 ... S64=11, D0=2, D2=6,
+ * ROW+BLK=3, DEV_GPU=4, UnaryEnum, BinaryEnum, ReductionEnum). */
+typedef int ma_datatype_t;
+typedef int ma_numdim_t;
+typedef int ma_memorder_t;
+typedef int ma_devicetype_t;
+typedef int ma_unary_t;
+typedef int ma_binary_t;
+typedef int ma_reduction_t;
+
+/* ---- devices and evaluation ------------------------------------------------ */
+
+/* front/bindings.hpp:27 ma_setupDevices. Selects the GPU (MAPX_DEVICE, default 0). */
+void ma_setupDevices(const char *plat_name, ma_devicetype_t dev, const char *dev_name);
+/* front/bindings.hpp:29-30 */
+void ma_increaseRef(ma_node_t *node);
+void ma_decreaseRef(ma_node_t *node);
+/* front/bindings.hpp:32 ma_eval: evaluates the nodes (fuse, compile, run). */
+void ma_eval(ma_node_t **vec, int num);
+/* front/bindings.hpp:33 ma_value: value of an evaluated D0 node. */
+ma_variant_t ma_value(ma_node_t *node);
+/* front/bindings.hpp:36-42 node attributes. */
+int ma_nodeid(ma_node_t *node);
+ma_datatype_t ma_datatype(ma_node_t *node);
+ma_numdim_t ma_numdim(ma_node_t *node);
+ma_memorder_t ma_memorder(ma_node_t *node);
+ma_dim2_t ma_datasize(ma_node_t *node);
+ma_dim2_t ma_blocksize(ma_node_t *node);
+
+/* ---- I/O --------------------------------------------------------------------- */
+
+/* front/bindings.hpp:48 ma_read: a .bin ROW+BLK raster (lazy). */
+ma_node_t *ma_read(const char *file_path);
+/* front/bindings.hpp:49 ma_write: evaluates and writes a .bin file; 0 on success. */
+int ma_write(ma_node_t *node, const char *file_path);
+
+/* ---- operations -------------------------------------------------------------- */
+
+/* front/bindings.hpp:55-60 */
+ma_node_t *ma_constant(ma_variant_t var, ma_dim2_t ds, ma_datatype_t dt, ma_memorder_t mo, ma_dim2_t bs);
+ma_node_t *ma_cast(ma_node_t *node, ma_datatype_t type);
+ma_node_t *ma_index(ma_node_t *node, ma_numdim_t dim);
+ma_node_t *ma_conditional(ma_node_t *cond, ma_node_t *lhs, ma_node_t *rhs);
+/* front/bindings.hpp:62-63 */
+ma_node_t *ma_unary(ma_node_t *node, ma_unary_t type);
+ma_node_t *ma_binary(ma_node_t *lhs, ma_node_t *rhs, ma_binary_t type);
+/* front/bindings.hpp:69 ma_neighbor */
+ma_node_t *ma_neighbor(ma_node_t *node, ma_dim2_t coord);
+/* front/bindings.hpp:72 ma_convolution: mask_s values, mask_ds = {ncols, nrows}, row-major. */
+ma_node_t *ma_convolution(ma_node_t *node, ma_variant_t *mask_v, int mask_s, ma_dim2_t mask_ds);
+/* front/bindings.hpp:73 ma_focalFunc */
+ma_node_t *ma_focalFunc(ma_node_t *node, ma_variant_t *mask_v, int mask_s, ma_dim2_t mask_ds, ma_reduction_t type);
+/* front/bindings.hpp:76 ma_zonalReduc: whole-raster reduction to a D0 value. */
+ma_node_t *ma_zonalReduc(ma_node_t *node, ma_reduction_t type);
+
+/* ---- extensions (no reference counterpart) ---------------------------------- */
+
+/* Description of the last failure of a ma_* call on this thread ("" if none). */
+const char *ma_lastError(void);
+/* Per-zone statistics of `data` over the integer `zone` raster, zones 0..nzones-1:
+ * a D1 node of nzones entries; after evaluation ma_zonalTable returns them. */
+ma_node_t *ma_zonalStats(ma_node_t *data, ma_node_t *zone, int nzones);
+/* Copies an evaluated per-zone table: FP64 sums, min, max (as double) and counts. */
+int ma_zonalTable(ma_node_t *node, double *sum, double *mn, double *mx, long long *count, int nzones);
+/* In-memory input raster: `host` holds the block-tiled payload (the .bin
+ * layout) of nbytes; the runtime copies it. */
+ma_node_t *ma_input(const void *host, size_t nbytes, ma_dim2_t ds, ma_datatype_t dt, ma_dim2_t bs);
+/* Copies the resident raster of an evaluated D2 node (block-tiled) to host. */
+int ma_fetch(ma_node_t *node, void *host, size_t nbytes);
+/* Releases the resident raster of a node (it is recomputed if evaluated again). */
+int ma_release(ma_node_t *node);
+/* Runtime option: "form" (0 plain: one launch per group per block, 1 batched),
+ * "graph" (replay launch sequences from hipGraphs), "focal_zonal" (fuse a focal
+ * group into the zonal group that consumes it), "tile_tx"/"tile_ty"/"tile_vx",
+ * "wg_per_cu". Returns 0 on success. */
+int ma_setOption(const char *key, long long value);
+long long ma_getOption(const char *key);
+/* Fusion groups and generated kernel sources of an evaluation, as JSON, without
+ * running it (valid until the next call on this thread). */
+const char *ma_plan(ma_node_t **vec, int num);
+/* Compiles every kernel of the plan with hiprtc for gfx950 (no device needed). */
+int ma_compilePlan(ma_node_t **vec, int num);
+/* Counters and device-memory figures of the runtime, as JSON. */
+const char *ma_stats(void);
+/* Synthetic DEM generated in HBM: ng Gaussians given as rows (cx, cy, sigma,
+ * amplitude) plus value-noise fBm with the given seed; F32, block-tiled. */
+ma_node_t *ma_genDem(ma_dim2_t ds, ma_dim2_t bs, const double *gauss, int ng, unsigned long long noise_seed);
+/* Voronoi zone raster of np integer points (px, py), type U8/U16/S32/U32. */
+ma_node_t *ma_genZones(ma_dim2_t ds, ma_dim2_t bs, ma_datatype_t dt, const int *px, const int *py, int np);
+/* Device time of the last evaluation in milliseconds (HIP events). */
+double ma_lastEvalMs(void);
+/* Fusion legality of a pattern pair (PatternEnum bit sets): bit 0 pipe, bit 1
+ * flat (runtime/Pattern.cpp:53-182 as applied by the fuser). */
+int ma_fusionRules(int top_or_left, int bot_or_right);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MAPX_H_ */

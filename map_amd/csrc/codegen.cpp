@@ -1,0 +1,474 @@
+#include "codegen.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <map>
+#include <set>
+#include <sstream>
+
+namespace mapx {
+
+namespace {
+
+// Group-local names: generated source must not depend on global node ids, so
+// that equal groups (e.g. every generation of an iterated stencil) share one
+// compiled kernel.
+thread_local const std::map<const Node*, int>* g_local = nullptr;
+std::string LN(const Node* n) {
+  auto it = g_local->find(n);
+  if (it == g_local->end()) throw std::runtime_error("codegen: node outside its group");
+  return std::to_string(it->second);
+}
+std::string V(const Node* n) { return "v" + LN(n); }
+const char* CT(int dt) { return ctype_dt(dt); }
+
+bool contains(const std::vector<Node*>& v, const Node* n) { return std::find(v.begin(), v.end(), n) != v.end(); }
+
+std::string hex_hash(const std::string& s) {
+  // FNV-1a 64: stable across processes (std::hash is not required to be).
+  uint64_t h = 1469598103934665603ull;
+  for (unsigned char c : s) { h ^= c; h *= 1099511628211ull; }
+  char buf[24];
+  std::snprintf(buf, sizeof buf, "%016llx", (unsigned long long)h);
+  return buf;
+}
+
+const char* red_op(int red) {
+  switch (red) {
+    case SUM: return "mx::OpSum"; case PROD: return "mx::OpProd"; case rAND: return "mx::OpAnd";
+    case rOR: return "mx::OpOr"; case MAX: return "mx::OpMax"; case MIN: return "mx::OpMin";
+  }
+  throw std::runtime_error("codegen: bad reduction");
+}
+
+// Accumulator type: floating SUM and PROD accumulate in FP64 (a fixed-order
+// sum of FP64 partials); every other reduction runs in the data type itself.
+int acc_type(int red, int dt) { return (is_float_dt(dt) && (red == SUM || red == PROD)) ? F64 : dt; }
+
+std::string lit(const Variant& v) { return variant_literal(v); }
+
+std::string unary_stmt(const Node* n, const std::string& p, int pdt) {
+  const int op = n->op, dt = n->meta.dt;
+  if (unary_is_operator(op)) return std::string("(") + CT(dt) + ")(" + unary_code(op) + p + ")";
+  if (op == ABS && !is_float_dt(dt)) return std::string("(") + CT(dt) + ")(" + p + " < 0 ? -" + p + " : " + p + ")";
+  std::string fn = op == ABS ? "fabs" : unary_code(op);
+  if (dt == F32) return fn + "f(" + p + ")";
+  if (dt == F64) return fn + "(" + p + ")";
+  (void)pdt;
+  return std::string("(") + CT(dt) + ")" + fn + "((double)" + p + ")";
+}
+
+std::string binary_stmt(const Node* n, const std::string& l, const std::string& r) {
+  const int op = n->op, dt = n->meta.dt;
+  const int ldt = n->prev[0]->meta.dt, rdt = n->prev[1]->meta.dt;
+  if (op == MOD && (is_float_dt(ldt) || is_float_dt(rdt))) {
+    int t = promote(ldt, rdt);
+    std::string f = t == F32 ? "fmodf" : "fmod";
+    return std::string("(") + CT(dt) + ")" + f + "((" + CT(t) + ")" + l + ", (" + CT(t) + ")" + r + ")";
+  }
+  if (binary_is_operator(op)) return std::string("(") + CT(dt) + ")(" + l + " " + binary_code(op) + " " + r + ")";
+  std::string L = std::string("((") + CT(dt) + ")" + l + ")", R = std::string("((") + CT(dt) + ")" + r + ")";
+  if (op == MAX2 || op == MIN2) {
+    if (dt == F32) return std::string(op == MAX2 ? "fmaxf(" : "fminf(") + L + ", " + R + ")";
+    if (dt == F64) return std::string(op == MAX2 ? "fmax(" : "fmin(") + L + ", " + R + ")";
+    return std::string("(") + L + (op == MAX2 ? " > " : " < ") + R + " ? " + L + " : " + R + ")";
+  }
+  std::string fn = binary_code(op);
+  if (dt == F32) return fn + "f(" + L + ", " + R + ")";
+  if (dt == F64) return fn + "(" + L + ", " + R + ")";
+  return std::string("(") + CT(dt) + ")" + fn + "((double)" + l + ", (double)" + r + ")";
+}
+
+}  // namespace
+
+Variant quantize_literal(const Variant& v) {
+  if (v.dtype == F32) {
+    std::string s = std::to_string((double)v.u.f32);
+    Variant r = v; r.u.f32 = (float)std::strtod(s.c_str(), nullptr); return r;
+  }
+  if (v.dtype == F64) {
+    std::string s = std::to_string(v.u.f64);
+    Variant r = v; r.u.f64 = std::strtod(s.c_str(), nullptr); return r;
+  }
+  return v;
+}
+
+TileShape default_tile(const Group& g) {
+  // cells per thread from the narrowest raster read or written (intermediates live in registers)
+  int minsz = 8;
+  for (Node* n : g.ins)
+    if (n->meta.nd == D2) minsz = std::min(minsz, sizeof_dt(n->meta.dt));
+  for (Node* n : g.nodes)
+    if (n->meta.nd == D2 && (std::find(g.outs.begin(), g.outs.end(), n) != g.outs.end() || n->next.empty()))
+      minsz = std::min(minsz, sizeof_dt(n->meta.dt));
+  if (minsz == 8) minsz = 4;
+  TileShape t;
+  if (minsz == 1) { t.VX = 4; t.TX = 256; t.TY = 16; }
+  else if (minsz == 2) { t.VX = 2; t.TX = 128; t.TY = 16; }
+  else { t.VX = 1; t.TX = 64; t.TY = 16; }
+  return t;
+}
+
+KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, TileShape tile) {
+  KernelSpec ks;
+  ks.tile = tile;
+  ks.d2_outs = d2_outs;
+  if (tile.TX % tile.VX) throw std::runtime_error("codegen: TX must be a multiple of VX");
+  const int TWX = tile.TX / tile.VX;
+  if (256 % TWX || tile.TY % (256 / TWX)) throw std::runtime_error("codegen: tile does not fit 256 threads");
+
+  // inputs first (D2 then D0, each by id), then the nodes by id: the local
+  // numbering is then the same for groups of equal structure
+  std::vector<Node*> all(g.ins.begin(), g.ins.end()), body(g.nodes.begin(), g.nodes.end());
+  std::sort(all.begin(), all.end(), [](Node* a, Node* b) {
+    if ((a->meta.nd == D0) != (b->meta.nd == D0)) return a->meta.nd != D0;
+    return a->id < b->id;
+  });
+  std::sort(body.begin(), body.end(), [](Node* a, Node* b) { return a->id < b->id; });
+  all.insert(all.end(), body.begin(), body.end());
+  std::set<Node*> in_set(g.ins.begin(), g.ins.end());
+  std::set<Node*> member(all.begin(), all.end());
+  std::map<const Node*, int> local;
+  for (size_t i = 0; i < all.size(); i++) local[all[i]] = (int)i;
+  for (Node* o : d2_outs)
+    if (!local.count(o)) throw std::runtime_error("codegen: output outside its group");
+  g_local = &local;
+
+  // ---- classification
+  std::vector<Node*> focal, staged, zonal;
+  for (Node* n : g.nodes) {
+    if (n->pattern() == FOCAL) focal.push_back(n);
+    if (n->kind == Kind::ZonalReduc || n->kind == Kind::ZonalStats) zonal.push_back(n);
+  }
+  for (Node* f : focal)
+    if (!contains(staged, f->prev[0])) staged.push_back(f->prev[0]);
+  std::sort(staged.begin(), staged.end(), [](Node* a, Node* b) { return a->id < b->id; });
+  for (Node* s : staged)
+    if (!member.count(s)) throw std::runtime_error("codegen: focal source outside its group");
+  // halo nodes: staged nodes and their ancestors inside the group
+  std::set<Node*> halo;
+  std::function<void(Node*)> up = [&](Node* n) {
+    if (!member.count(n) || !halo.insert(n).second) return;
+    if (in_set.count(n)) return;
+    for (Node* p : n->prev) up(p);
+  };
+  for (Node* s : staged) up(s);
+  for (Node* n : halo)
+    if (n->pattern() == FOCAL) throw std::runtime_error("codegen: focal node feeding a focal node in one group");
+
+  ks.form = focal.empty() ? Form::Local : Form::Focal;
+  for (Node* f : focal) { ks.H0 = std::max(ks.H0, f->halo(0)); ks.H1 = std::max(ks.H1, f->halo(1)); }
+
+  // cell nodes: everything computed per output cell
+  std::vector<Node*> cell;
+  for (Node* n : g.nodes) if (!halo.count(n)) cell.push_back(n);
+  // halo values needed at the centre cell (by non-focal cell nodes or as outputs)
+  std::set<Node*> centre;
+  for (Node* n : cell) {
+    if (n->pattern() == FOCAL) continue;
+    for (Node* p : n->prev) if (halo.count(p)) centre.insert(p);
+  }
+  for (Node* o : d2_outs) if (halo.count(o)) centre.insert(o);
+  {
+    std::vector<Node*> stack(centre.begin(), centre.end());
+    while (!stack.empty()) {
+      Node* n = stack.back(); stack.pop_back();
+      if (in_set.count(n) || contains(staged, n)) continue;
+      for (Node* p : n->prev)
+        if (halo.count(p) && centre.insert(p).second) stack.push_back(p);
+    }
+  }
+  // inputs read by cell nodes directly
+  std::set<Node*> cell_inputs;
+  for (Node* n : cell)
+    for (Node* p : n->prev)
+      if (in_set.count(p) && !halo.count(p)) cell_inputs.insert(p);
+  for (Node* o : d2_outs) if (in_set.count(o) && !halo.count(o)) cell_inputs.insert(o);
+
+  // ---- parameters
+  for (Node* n : all) {
+    if (!in_set.count(n)) continue;
+    Param p;
+    p.kind = n->meta.nd == D0 ? Param::InD0 : Param::InD2;
+    p.node = n; p.dt = n->meta.dt;
+    ks.params.push_back(p);
+  }
+  for (Node* n : d2_outs) { Param p; p.kind = Param::OutD2; p.node = n; p.dt = n->meta.dt; ks.params.push_back(p); }
+  size_t part_bytes = 0, result_bytes = 0;
+  std::vector<size_t> part_off;
+  for (Node* z : zonal) {
+    ZonalOut zo{};
+    zo.node = z;
+    zo.dt = z->prev[0]->meta.dt;
+    if (z->kind == Kind::ZonalReduc) {
+      zo.red = z->op;
+      zo.acc_dt = acc_type(z->op, zo.dt);
+      part_off.push_back(part_bytes);
+      part_bytes += 8;
+      zo.result_off = result_bytes;
+      result_bytes += 8;
+    } else {
+      if (!(zo.dt == F32 || zo.dt == S32 || zo.dt == U32 || zo.dt == U8 || zo.dt == U16 || zo.dt == S8 || zo.dt == S16))
+        throw std::runtime_error("codegen: per-zone statistics support 8/16/32-bit data");
+      zo.red = 0;
+      zo.acc_dt = F64;
+      zo.nzones = z->nzones;
+      part_off.push_back(part_bytes);
+      part_bytes += (size_t)z->nzones * 24;
+      zo.result_off = result_bytes;
+      result_bytes += (size_t)z->nzones * 24;
+    }
+    ks.zouts.push_back(zo);
+  }
+  ks.part_bytes = part_bytes;
+  ks.result_bytes = result_bytes;
+  if (!zonal.empty()) { Param p; p.kind = Param::Partials; ks.params.push_back(p); }
+  { Param p; p.kind = Param::Launch; ks.params.push_back(p); }
+
+  // the kernel is named after the hash of its own source (with the name
+  // replaced by a placeholder), which is also the compile-cache key
+  ks.name = "KRN_NAME";
+  ks.fin_name = zonal.empty() ? "" : "KRN_NAME_fin";
+
+  // ---- source
+  std::ostringstream s;
+  const int HW = tile.TX + 2 * ks.H0, HH = tile.TY + 2 * ks.H1;
+  s << "#include \"mapx_dev.hiph\"\n";
+  s << "namespace {\nstruct P {\n";
+  s << "  static constexpr int H0 = " << ks.H0 << ", H1 = " << ks.H1 << ", SW = " << HW << ";\n";
+  s << "  struct Args {\n";
+  for (const Param& p : ks.params) {
+    if (p.kind == Param::InD2) s << "    mx::In<" << CT(p.dt) << "> i" << LN(p.node) << ";\n";
+    if (p.kind == Param::InD0) s << "    " << CT(p.dt) << " s" << LN(p.node) << ";\n";
+    if (p.kind == Param::OutD2) s << "    " << CT(p.dt) << "* __restrict__ o" << LN(p.node) << ";\n";
+  }
+  s << "  };\n";
+  s << "  struct Smem {\n";
+  for (Node* st : staged) s << "    " << CT(st->meta.dt) << " sh" << LN(st) << "[" << HW * HH << "];\n";
+  if (staged.empty()) s << "    int unused;\n";
+  s << "  };\n";
+  // accumulators
+  s << "  struct Acc {\n";
+  for (size_t j = 0; j < ks.zouts.size(); j++) {
+    const ZonalOut& z = ks.zouts[j];
+    if (z.node->kind == Kind::ZonalReduc) s << "    " << CT(z.acc_dt) << " r" << j << ";\n";
+    else s << "    mx::ZoneLds<" << CT(z.dt) << "> z" << j << ";\n";
+  }
+  s << "    int unused;\n  };\n";
+
+  auto emit_node = [&](std::ostream& o, Node* n, bool in_pre) {
+    const std::string T = CT(n->meta.dt);
+    switch (n->kind) {
+      case Kind::Constant:
+        o << "    const " << T << " " << V(n) << " = " << lit(quantize_literal(n->cnst)) << ";\n"; break;
+      case Kind::Index:
+        o << "    const " << T << " " << V(n) << " = (" << T << ")("
+          << (n->op == D1 ? "k.bx * k.BS0 + x" : "k.by * k.BS1 + y") << ");\n"; break;
+      case Kind::Cast:
+        o << "    const " << T << " " << V(n) << " = (" << T << ")" << V(n->prev[0]) << ";\n"; break;
+      case Kind::Unary:
+        o << "    const " << T << " " << V(n) << " = " << unary_stmt(n, V(n->prev[0]), n->prev[0]->meta.dt) << ";\n"; break;
+      case Kind::Binary:
+        o << "    const " << T << " " << V(n) << " = " << binary_stmt(n, V(n->prev[0]), V(n->prev[1])) << ";\n"; break;
+      case Kind::Conditional:
+        o << "    const " << T << " " << V(n) << " = (" << V(n->prev[0]) << ") ? " << V(n->prev[1]) << " : "
+          << V(n->prev[2]) << ";\n"; break;
+      case Kind::Neighbor:
+        o << "    const " << T << " " << V(n) << " = sm->sh" << LN(n->prev[0]) << "[si + (" << n->coord.arr[1]
+          << ") * SW + (" << n->coord.arr[0] << ")];\n"; break;
+      case Kind::Convolution: {
+        // acc = 0; for y (outer), x (inner): acc += S[y][x] * W[y][x]  (FocalSkeleton.cpp:300-336)
+        o << "    " << T << " " << V(n) << " = 0;\n";
+        const int h0 = n->halo(0), h1 = n->halo(1);
+        for (int i1 = -h1; i1 <= h1; i1++)
+          for (int i0 = -h0; i0 <= h0; i0++) {
+            Variant w = quantize_literal(n->mask.at(i0 + h0, i1 + h1));
+            o << "    " << V(n) << " += sm->sh" << LN(n->prev[0]) << "[si + (" << i1 << ") * SW + (" << i0 << ")] * "
+              << lit(w) << ";\n";
+          }
+        break;
+      }
+      case Kind::FocalFunc: {
+        Variant neu = reduction_neutral(n->op, n->meta.dt);
+        o << "    " << T << " " << V(n) << " = " << lit(neu) << ";\n";
+        const int h0 = n->halo(0), h1 = n->halo(1);
+        for (int i1 = -h1; i1 <= h1; i1++)
+          for (int i0 = -h0; i0 <= h0; i0++) {
+            std::string sv = "sm->sh" + LN(n->prev[0]) + "[si + (" + std::to_string(i1) + ") * SW + (" +
+                             std::to_string(i0) + ")]";
+            if (n->op == MAX || n->op == MIN) {
+              if (n->meta.dt == F32) o << "    " << V(n) << " = " << (n->op == MAX ? "fmaxf(" : "fminf(") << V(n) << ", " << sv << ");\n";
+              else if (n->meta.dt == F64) o << "    " << V(n) << " = " << (n->op == MAX ? "fmax(" : "fmin(") << V(n) << ", " << sv << ");\n";
+              else o << "    { const " << T << " t_ = " << sv << "; " << V(n) << " = (t_ " << (n->op == MAX ? ">" : "<") << " "
+                     << V(n) << ") ? t_ : " << V(n) << "; }\n";
+            } else {
+              const char* c = n->op == SUM ? "+" : n->op == PROD ? "*" : n->op == rAND ? "&" : "|";
+              o << "    " << V(n) << " = (" << T << ")(" << V(n) << " " << c << " " << sv << ");\n";
+            }
+          }
+        break;
+      }
+      case Kind::ZonalReduc: case Kind::ZonalStats: break;  // accumulated below
+      default: throw std::runtime_error(std::string("codegen: unexpected node kind ") + kind_name(n->kind));
+    }
+    (void)in_pre;
+  };
+
+  // pre(): values of the staged nodes at one halo cell
+  s << "  __device__ static __forceinline__ void pre(const Args& a, const mx::Blk& k, int x, int y, bool inner, Smem& sm, int si) {\n";
+  for (Node* n : all) {
+    if (!halo.count(n)) continue;
+    if (in_set.count(n)) {
+      if (n->meta.nd == D0) s << "    const " << CT(n->meta.dt) << " " << V(n) << " = a.s" << LN(n) << ";\n";
+      else s << "    const " << CT(n->meta.dt) << " " << V(n) << " = inner ? mx::ldl(a.i" << LN(n) << ", k, x, y) : mx::ldf(a.i"
+             << LN(n) << ", k, x, y);\n";
+    } else emit_node(s, n, true);
+  }
+  for (Node* st : staged) s << "    sm.sh" << LN(st) << "[si] = " << V(st) << ";\n";
+  s << "    (void)a; (void)k; (void)x; (void)y; (void)inner; (void)sm; (void)si;\n  }\n";
+  s << "  __device__ static __forceinline__ void pre_zero(Smem& sm, int si) {\n";
+  for (Node* st : staged) s << "    sm.sh" << LN(st) << "[si] = 0;\n";
+  s << "    (void)sm; (void)si;\n  }\n";
+
+  // post(): one output cell
+  s << "  __device__ static __forceinline__ void post(const Args& a, const mx::Blk& k, int x, int y, bool ok, const Smem* sm, int si, Acc& acc) {\n";
+  for (Node* n : all) {
+    if (halo.count(n)) {
+      if (!centre.count(n)) continue;
+      if (contains(staged, n)) s << "    const " << CT(n->meta.dt) << " " << V(n) << " = sm->sh" << LN(n) << "[si];\n";
+      else if (in_set.count(n)) {
+        if (n->meta.nd == D0) s << "    const " << CT(n->meta.dt) << " " << V(n) << " = a.s" << LN(n) << ";\n";
+        else s << "    const " << CT(n->meta.dt) << " " << V(n) << " = mx::ldl(a.i" << LN(n) << ", k, x, y);\n";
+      } else emit_node(s, n, false);
+      continue;
+    }
+    if (in_set.count(n)) {
+      if (!cell_inputs.count(n)) continue;
+      if (n->meta.nd == D0) s << "    const " << CT(n->meta.dt) << " " << V(n) << " = a.s" << LN(n) << ";\n";
+      else s << "    const " << CT(n->meta.dt) << " " << V(n) << " = mx::ldl(a.i" << LN(n) << ", k, x, y);\n";
+      continue;
+    }
+    emit_node(s, n, false);
+  }
+  for (Node* o : d2_outs)
+    s << "    if (ok) a.o" << LN(o) << "[k.off + (mx::i64)y * k.BS0 + x] = " << V(o) << ";\n";
+  for (size_t j = 0; j < ks.zouts.size(); j++) {
+    const ZonalOut& z = ks.zouts[j];
+    Node* src = z.node->prev[0];
+    if (z.node->kind == Kind::ZonalReduc)
+      s << "    if (ok) acc.r" << j << " = " << red_op(z.red) << "::f(acc.r" << j << ", (" << CT(z.acc_dt) << ")" << V(src) << ");\n";
+    else
+      s << "    mx::zone_step<" << CT(z.dt) << ", " << z.nzones << ">(acc.z" << j << ", ok, " << V(src) << ", (int)"
+        << V(z.node->prev[1]) << ");\n";
+  }
+  s << "    (void)a; (void)k; (void)x; (void)y; (void)ok; (void)sm; (void)si; (void)acc;\n  }\n";
+  s << "};\n}  // namespace\n\n";
+
+  // main kernel
+  s << "extern \"C\" __global__ void __launch_bounds__(256) " << ks.name << "(";
+  bool first = true;
+  auto comma = [&]() { if (!first) s << ", "; first = false; };
+  for (const Param& p : ks.params) {
+    comma();
+    switch (p.kind) {
+      case Param::InD2: s << "mx::In<" << CT(p.dt) << "> i" << LN(p.node); break;
+      case Param::InD0: s << CT(p.dt) << " s" << LN(p.node); break;
+      case Param::OutD2: s << CT(p.dt) << "* o" << LN(p.node); break;
+      case Param::Partials: s << "char* zpart"; break;
+      case Param::Launch: s << "mx::Launch L"; break;
+    }
+  }
+  s << ") {\n";
+  s << "  P::Args a = {";
+  first = true;
+  for (const Param& p : ks.params) {
+    if (p.kind == Param::InD2) { comma(); s << "i" << LN(p.node); }
+    if (p.kind == Param::InD0) { comma(); s << "s" << LN(p.node); }
+    if (p.kind == Param::OutD2) { comma(); s << "o" << LN(p.node); }
+  }
+  s << "};\n";
+  s << "  P::Acc acc;\n";
+  for (size_t j = 0; j < ks.zouts.size(); j++) {
+    const ZonalOut& z = ks.zouts[j];
+    if (z.node->kind == Kind::ZonalReduc) {
+      s << "  acc.r" << j << " = " << lit(reduction_neutral(z.red, z.acc_dt)) << ";\n";
+    } else {
+      std::string M = std::string("mx::MM<") + CT(z.dt) + ">::t";
+      s << "  __shared__ double zs" << j << "[4 * " << z.nzones << "];\n";
+      s << "  __shared__ " << M << " zn" << j << "[" << z.nzones << "], zx" << j << "[" << z.nzones << "];\n";
+      s << "  __shared__ mx::u32 zc" << j << "[" << z.nzones << "];\n";
+      s << "  acc.z" << j << ".sum = zs" << j << "; acc.z" << j << ".mn = zn" << j << "; acc.z" << j << ".mx = zx" << j
+        << "; acc.z" << j << ".cnt = zc" << j << ";\n";
+      s << "  mx::zone_init<" << CT(z.dt) << ", " << z.nzones << ">(acc.z" << j << ");\n";
+    }
+  }
+  if (!zonal.empty()) s << "  __syncthreads();\n";
+  if (ks.form == Form::Focal) {
+    s << "  __shared__ P::Smem sm;\n";
+    s << "  mx::focal_body<P, " << tile.TX << ", " << tile.TY << ", " << tile.VX << ">(a, L, acc, sm);\n";
+  } else {
+    s << "  mx::local_body<P, " << tile.TX << ", " << tile.TY << ", " << tile.VX << ">(a, L, acc);\n";
+  }
+  if (!zonal.empty()) {
+    s << "  __shared__ mx::u64 red_scratch[4];\n";
+    s << "  char* my = zpart + (size_t)(L.part0 + blockIdx.x) * " << part_bytes << "u;\n";
+    for (size_t j = 0; j < ks.zouts.size(); j++) {
+      const ZonalOut& z = ks.zouts[j];
+      if (z.node->kind == Kind::ZonalReduc) {
+        std::string A = CT(z.acc_dt);
+        s << "  mx::block_reduce_store<" << red_op(z.red) << ", " << A << ">(acc.r" << j << ", (" << A << "*)(my + " << part_off[j]
+          << "), (" << A << "*)red_scratch);\n";
+      } else {
+        s << "  mx::zone_flush<" << CT(z.dt) << ", " << z.nzones << ">(acc.z" << j << ", (mx::ZonePart<" << CT(z.dt)
+          << ">*)(my + " << part_off[j] << "));\n";
+      }
+    }
+  }
+  s << "}\n";
+
+  // finalisation kernel: folds the partials of every work-group in a fixed order
+  if (!zonal.empty()) {
+    s << "\nextern \"C\" __global__ void __launch_bounds__(256) " << ks.fin_name
+      << "(const char* zpart, int nparts, char* res) {\n";
+    s << "  __shared__ mx::u64 red_scratch[4];\n";
+    for (size_t j = 0; j < ks.zouts.size(); j++) {
+      const ZonalOut& z = ks.zouts[j];
+      if (z.node->kind == Kind::ZonalReduc) {
+        std::string A = CT(z.acc_dt);
+        s << "  {\n    " << A << " r = " << lit(reduction_neutral(z.red, z.acc_dt)) << ";\n";
+        s << "    for (int p = threadIdx.x; p < nparts; p += 256) r = " << red_op(z.red) << "::f(r, *(const " << A
+          << "*)(zpart + (size_t)p * " << part_bytes << "u + " << part_off[j] << "));\n";
+        s << "    mx::block_reduce_store<" << red_op(z.red) << ", " << A << ">(r, (" << A << "*)(res + " << z.result_off
+          << "), (" << A << "*)red_scratch);\n  }\n";
+      } else {
+        std::string T = CT(z.dt);
+        std::string M = std::string("mx::MM<") + T + ">::t";
+        s << "  for (int zi = threadIdx.x; zi < " << z.nzones << "; zi += 256) {\n";
+        s << "    double sum = 0.0; " << M << " mn = mx::mm_highest<" << T << ">(), mxv = mx::mm_lowest<" << T
+          << ">(); mx::u32 cnt = 0u;\n";
+        s << "    for (int p = 0; p < nparts; p++) {\n";
+        s << "      const mx::ZonePart<" << T << ">& q = ((const mx::ZonePart<" << T << ">*)(zpart + (size_t)p * "
+          << part_bytes << "u + " << part_off[j] << "))[zi];\n";
+        s << "      sum += q.sum; mn = mx::OpMin::f(mn, q.mn); mxv = mx::OpMax::f(mxv, q.mx); cnt += q.cnt;\n    }\n";
+        s << "    mx::ZonePart<" << T << "> r; r.sum = sum; r.mn = mn; r.mx = mxv; r.cnt = cnt; r.pad = 0u;\n";
+        s << "    ((mx::ZonePart<" << T << ">*)(res + " << z.result_off << "))[zi] = r;\n  }\n";
+      }
+    }
+    s << "}\n";
+  }
+  std::string src = s.str();
+  const std::string h = hex_hash(src);
+  auto subst = [&](std::string t) {
+    for (size_t pos; (pos = t.find("KRN_NAME")) != std::string::npos;) t.replace(pos, 8, "krn_" + h);
+    return t;
+  };
+  ks.source = subst(src);
+  ks.name = "krn_" + h;
+  ks.fin_name = zonal.empty() ? "" : ks.name + "_fin";
+  ks.signature = ks.source;
+  g_local = nullptr;
+  return ks;
+}
+
+}  // namespace mapx

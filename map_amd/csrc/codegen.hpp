@@ -1,0 +1,67 @@
+// Code generation: one fused group -> HIP source instantiating the templates
+// of device/mapx_dev.hiph, compiled by hiprtc for gfx950.
+//
+// Replaces the reference's OpenCL skeleton generators
+// (runtime/skeleton/{Skeleton,LocalSkeleton,FocalSkeleton,ZonalSkeleton}.cpp):
+// node statements are still emitted one per node in node-id order with the
+// node's promoted type, but the loops, tiling, loads and reductions are the
+// hand-written templates.
+#pragma once
+#include "fuser.hpp"
+#include <string>
+#include <vector>
+
+namespace mapx {
+
+enum class Form : int { Local = 0, Focal = 1 };
+
+// One kernel parameter (in declaration order).
+struct Param {
+  enum Kind : int { InD2, InD0, OutD2, Partials, Launch } kind;
+  Node* node = nullptr;  // InD2/InD0/OutD2
+  int dt = 0;
+};
+
+// A zonal output of the group.
+struct ZonalOut {
+  Node* node;      // ZonalReduc or ZonalStats
+  int red;         // reduction (ZonalReduc) or 0 (ZonalStats)
+  int dt;          // data type of the reduced values
+  int acc_dt;      // accumulator type (F64 for floating SUM/PROD)
+  int nzones;      // ZonalStats only
+  size_t result_off;  // byte offset of the final value(s) in the result buffer
+};
+
+struct TileShape {
+  int TX = 64, TY = 16, VX = 1;
+  bool operator==(const TileShape& o) const { return TX == o.TX && TY == o.TY && VX == o.VX; }
+};
+
+struct KernelSpec {
+  std::string name, fin_name;  // main kernel; zonal finalisation kernel ("" if none)
+  std::string source;
+  Form form = Form::Local;
+  TileShape tile;
+  int H0 = 0, H1 = 0;
+  std::vector<Param> params;
+  std::vector<Node*> d2_outs;     // materialised D2 outputs, in parameter order
+  std::vector<ZonalOut> zouts;
+  size_t part_bytes = 0;          // bytes of one work-group's partial record
+  size_t result_bytes = 0;        // bytes of the finalised results
+  std::string signature;          // cache key (group signature + tile shape)
+};
+
+// Builds the kernel of group `g`. `d2_outs` are the D2 nodes the group must
+// write to HBM (group outputs consumed later, written to files or requested).
+KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, TileShape tile);
+
+// Default tile shape for a group: VX cells per thread along x so that a
+// wavefront row covers >= 256 contiguous bytes.
+TileShape default_tile(const Group& g);
+
+// Reference quantisation of floating mask weights and D2 constant literals:
+// printed with std::to_string (%f, 6 decimals) into the kernel source
+// (runtime/skeleton/util.cpp:136-165, util/VariantType.cpp:91) and read back.
+Variant quantize_literal(const Variant& v);
+
+}  // namespace mapx

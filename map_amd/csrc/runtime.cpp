@@ -1,0 +1,809 @@
+#include "runtime.hpp"
+#include "dev.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+
+namespace mapx {
+
+static void fail(const std::string& m) { throw std::runtime_error(m); }
+
+// ---- Store ---------------------------------------------------------------------
+
+void* Store::alloc(size_t bytes) {
+  bytes = (bytes + 255) & ~size_t(255);
+  auto it = free_.find(bytes);
+  void* p;
+  if (it != free_.end()) {
+    p = it->second;
+    free_.erase(it);
+    cached_ -= bytes;
+  } else {
+    try {
+      p = dev::malloc_device(bytes);
+    } catch (const std::exception&) {
+      trim();  // give cached buffers back and retry once
+      p = dev::malloc_device(bytes);
+    }
+  }
+  size_[p] = bytes;
+  in_use_ += bytes;
+  return p;
+}
+
+void Store::release(void* p) {
+  auto it = size_.find(p);
+  if (it == size_.end()) fail("store: release of an unknown buffer");
+  free_.emplace(it->second, p);
+  cached_ += it->second;
+  in_use_ -= it->second;
+  size_.erase(it);
+}
+
+void Store::trim() {
+  if (!free_.empty()) dev::sync();
+  for (auto& f : free_) dev::free_device(f.second);
+  free_.clear();
+  cached_ = 0;
+}
+
+Store::~Store() {
+  // process exit: the device context may already be gone; leave buffers to the driver
+}
+
+// ---- .bin file format -------------------------------------------------------------
+
+static const size_t kPage = 4096;
+
+BinHeader bin_read_header(const std::string& path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) fail("read: cannot open " + path);
+  int32_t h[7];
+  f.read((char*)h, sizeof h);
+  if (!f) fail("read: short header in " + path);
+  BinHeader b{h[0], h[1], h[2], h[3], h[4], h[5], h[6]};
+  if (b.dt <= NONE_DATATYPE || b.dt >= N_DATATYPE) fail("read: bad data type in " + path);
+  if (b.nd != D2) fail("read: only D2 rasters are supported: " + path);
+  if (b.mo != (ROW + BLK)) fail("read: memory order must be ROW+BLK: " + path);
+  if (b.ds0 <= 0 || b.ds1 <= 0 || b.bs0 <= 0 || b.bs1 <= 0) fail("read: bad sizes in " + path);
+  return b;
+}
+
+void bin_read_payload(const std::string& path, void* dst, size_t bytes) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) fail("read: cannot open " + path);
+  f.seekg((std::streamoff)kPage);
+  f.read((char*)dst, (std::streamsize)bytes);
+  if ((size_t)f.gcount() != bytes) fail("read: short payload in " + path);
+}
+
+void bin_write(const std::string& path, const Meta& m, const void* payload, size_t bytes) {
+  std::ofstream f(path, std::ios::binary | std::ios::trunc);
+  if (!f) fail("write: cannot open " + path);
+  std::vector<char> page(kPage, 0);
+  int32_t h[7] = {m.dt, m.nd, m.ds.arr[0], m.ds.arr[1], m.mo, m.bs.arr[0], m.bs.arr[1]};
+  std::memcpy(page.data(), h, sizeof h);
+  f.write(page.data(), (std::streamsize)page.size());
+  f.write((const char*)payload, (std::streamsize)bytes);
+  if (!f) fail("write: I/O error on " + path);
+}
+
+Node* Dag::read(const std::string& path) {
+  BinHeader b = bin_read_header(path);
+  auto n = std::make_unique<Node>();
+  n->kind = Kind::Read;
+  n->meta.dt = b.dt; n->meta.nd = D2; n->meta.mo = b.mo;
+  n->meta.ds = dim2(b.ds0, b.ds1);
+  n->meta.bs = dim2(b.bs0, b.bs1);
+  n->path = path;
+  return add(std::move(n));
+}
+
+// ---- host scalar evaluation with C semantics -----------------------------------------
+
+namespace {
+enum CK { CI32, CU32, CI64, CU64, CF32, CF64 };
+struct CV {
+  CK k;
+  int64_t i = 0;
+  uint64_t u = 0;
+  double d = 0;
+  float f = 0;
+};
+CV to_cv(const Variant& v) {  // integer promotion of the stored type
+  CV c;
+  switch (v.dtype) {
+    case B8: c.k = CI32; c.i = v.u.b8; break;
+    case U8: c.k = CI32; c.i = v.u.u8; break;
+    case S8: c.k = CI32; c.i = v.u.s8; break;
+    case U16: c.k = CI32; c.i = v.u.u16; break;
+    case S16: c.k = CI32; c.i = v.u.s16; break;
+    case S32: c.k = CI32; c.i = v.u.s32; break;
+    case U32: c.k = CU32; c.u = v.u.u32; break;
+    case S64: c.k = CI64; c.i = v.u.s64; break;
+    case U64: c.k = CU64; c.u = v.u.u64; break;
+    case F32: c.k = CF32; c.f = v.u.f32; break;
+    case F64: c.k = CF64; c.d = v.u.f64; break;
+    default: fail("scalar: value without a type");
+  }
+  return c;
+}
+int rank(CK k) { return (k == CI64 || k == CU64) ? 2 : 1; }
+bool is_uns(CK k) { return k == CU32 || k == CU64; }
+CK common(CK a, CK b) {  // usual arithmetic conversions
+  if (a == CF64 || b == CF64) return CF64;
+  if (a == CF32 || b == CF32) return CF32;
+  if (a == b) return a;
+  if (is_uns(a) == is_uns(b)) return rank(a) >= rank(b) ? a : b;
+  CK u = is_uns(a) ? a : b, s = is_uns(a) ? b : a;
+  if (rank(u) >= rank(s)) return u;
+  return s;  // I64 represents every U32
+}
+CV conv(const CV& c, CK k) {
+  CV r; r.k = k;
+  auto as_i = [&]() -> int64_t {
+    switch (c.k) { case CI32: case CI64: return c.i; case CU32: case CU64: return (int64_t)c.u;
+      case CF32: return (int64_t)c.f; default: return (int64_t)c.d; }
+  };
+  auto as_u = [&]() -> uint64_t {
+    switch (c.k) { case CI32: case CI64: return (uint64_t)c.i; case CU32: case CU64: return c.u;
+      case CF32: return (uint64_t)c.f; default: return (uint64_t)c.d; }
+  };
+  auto as_d = [&]() -> double {
+    switch (c.k) { case CI32: case CI64: return (double)c.i; case CU32: case CU64: return (double)c.u;
+      case CF32: return (double)c.f; default: return c.d; }
+  };
+  switch (k) {
+    case CI32: r.i = (int32_t)as_i(); break;
+    case CI64: r.i = as_i(); break;
+    case CU32: r.u = (uint32_t)as_u(); break;
+    case CU64: r.u = as_u(); break;
+    case CF32: r.f = (c.k == CF32) ? c.f : (c.k == CI32 || c.k == CI64) ? (float)c.i
+                   : (c.k == CU32 || c.k == CU64) ? (float)c.u : (float)c.d; break;
+    case CF64: r.d = as_d(); break;
+  }
+  return r;
+}
+Variant from_cv(const CV& c, int dt) {
+  Variant v;
+  switch (c.k) {
+    case CI32: v.dtype = S32; v.u.s32 = (int32_t)c.i; break;
+    case CI64: v.dtype = S64; v.u.s64 = c.i; break;
+    case CU32: v.dtype = U32; v.u.u32 = (uint32_t)c.u; break;
+    case CU64: v.dtype = U64; v.u.u64 = c.u; break;
+    case CF32: v.dtype = F32; v.u.f32 = c.f; break;
+    case CF64: v.dtype = F64; v.u.f64 = c.d; break;
+  }
+  return variant_convert(v, dt);
+}
+CV cv_int(int64_t x) { CV c; c.k = CI32; c.i = x; return c; }
+
+CV binop(int op, CV a, CV b) {
+  CK k = common(a.k, b.k);
+  if (op == SHL || op == SHR) k = a.k;  // shifts take the promoted left type
+  CV x = conv(a, k), y = conv(b, op == SHL || op == SHR ? (rank(b.k) == 2 ? CI64 : CI32) : k), r;
+  r.k = k;
+  auto cmp = [&](int c) { return cv_int(c); };
+  if (k == CF32 || k == CF64) {
+    double l = k == CF32 ? x.f : x.d, rr = k == CF32 ? y.f : y.d;
+    double res = 0;
+    switch (op) {
+      case ADD: res = k == CF32 ? (double)(x.f + y.f) : l + rr; break;
+      case SUB: res = k == CF32 ? (double)(x.f - y.f) : l - rr; break;
+      case MUL: res = k == CF32 ? (double)(x.f * y.f) : l * rr; break;
+      case DIV: res = k == CF32 ? (double)(x.f / y.f) : l / rr; break;
+      case EQ: return cmp(l == rr); case NE: return cmp(l != rr); case LT: return cmp(l < rr);
+      case GT: return cmp(l > rr); case LE: return cmp(l <= rr); case GE: return cmp(l >= rr);
+      case AND: return cmp(l != 0 && rr != 0); case OR: return cmp(l != 0 || rr != 0);
+      default: fail("scalar: operator not defined on floating types");
+    }
+    if (k == CF32) r.f = (float)res; else r.d = res;
+    return r;
+  }
+  if (is_uns(k)) {
+    uint64_t l = x.u, rr = y.u, res = 0;
+    uint64_t mask = k == CU32 ? 0xffffffffull : ~0ull;
+    switch (op) {
+      case ADD: res = l + rr; break; case SUB: res = l - rr; break; case MUL: res = l * rr; break;
+      case DIV: if (!rr) fail("scalar: integer division by zero"); res = l / rr; break;
+      case MOD: if (!rr) fail("scalar: integer modulo by zero"); res = l % rr; break;
+      case EQ: return cmp(l == rr); case NE: return cmp(l != rr); case LT: return cmp(l < rr);
+      case GT: return cmp(l > rr); case LE: return cmp(l <= rr); case GE: return cmp(l >= rr);
+      case AND: return cmp(l && rr); case OR: return cmp(l || rr);
+      case bAND: res = l & rr; break; case bOR: res = l | rr; break; case bXOR: res = l ^ rr; break;
+      case SHL: res = l << (y.i & (k == CU32 ? 31 : 63)); break; case SHR: res = l >> (y.i & (k == CU32 ? 31 : 63)); break;
+      default: fail("scalar: bad operator");
+    }
+    r.u = res & mask;
+    return r;
+  }
+  int64_t l = x.i, rr = y.i, res = 0;
+  switch (op) {
+    case ADD: res = (int64_t)((uint64_t)l + (uint64_t)rr); break;
+    case SUB: res = (int64_t)((uint64_t)l - (uint64_t)rr); break;
+    case MUL: res = (int64_t)((uint64_t)l * (uint64_t)rr); break;
+    case DIV: if (!rr) fail("scalar: integer division by zero"); res = l / rr; break;
+    case MOD: if (!rr) fail("scalar: integer modulo by zero"); res = l % rr; break;
+    case EQ: return cmp(l == rr); case NE: return cmp(l != rr); case LT: return cmp(l < rr);
+    case GT: return cmp(l > rr); case LE: return cmp(l <= rr); case GE: return cmp(l >= rr);
+    case AND: return cmp(l && rr); case OR: return cmp(l || rr);
+    case bAND: res = l & rr; break; case bOR: res = l | rr; break; case bXOR: res = l ^ rr; break;
+    case SHL: res = (int64_t)((uint64_t)l << (rr & 63)); break; case SHR: res = l >> (rr & 63); break;
+    default: fail("scalar: bad operator");
+  }
+  if (k == CI32) res = (int32_t)res;
+  r.i = res;
+  return r;
+}
+
+double fn1(int op, double x) {
+  switch (op) {
+    case SIN: return std::sin(x); case COS: return std::cos(x); case TAN: return std::tan(x);
+    case ASIN: return std::asin(x); case ACOS: return std::acos(x); case ATAN: return std::atan(x);
+    case SINH: return std::sinh(x); case COSH: return std::cosh(x); case TANH: return std::tanh(x);
+    case ASINH: return std::asinh(x); case ACOSH: return std::acosh(x); case ATANH: return std::atanh(x);
+    case EXP: return std::exp(x); case EXP2: return std::exp2(x); case EXP10: return std::pow(10.0, x);
+    case LOG: return std::log(x); case LOG2: return std::log2(x); case LOG10: return std::log10(x);
+    case SQRT: return std::sqrt(x); case CBRT: return std::cbrt(x); case ABS: return std::fabs(x);
+    case CEIL: return std::ceil(x); case FLOOR: return std::floor(x); case TRUNC: return std::trunc(x);
+    case ROUND: return std::round(x);
+  }
+  fail("scalar: bad function");
+  return 0;
+}
+float fn1f(int op, float x) {
+  switch (op) {
+    case SIN: return std::sin(x); case COS: return std::cos(x); case TAN: return std::tan(x);
+    case ASIN: return std::asin(x); case ACOS: return std::acos(x); case ATAN: return std::atan(x);
+    case SINH: return std::sinh(x); case COSH: return std::cosh(x); case TANH: return std::tanh(x);
+    case ASINH: return std::asinh(x); case ACOSH: return std::acosh(x); case ATANH: return std::atanh(x);
+    case EXP: return std::exp(x); case EXP2: return std::exp2(x); case EXP10: return (float)std::pow(10.0, (double)x);
+    case LOG: return std::log(x); case LOG2: return std::log2(x); case LOG10: return std::log10(x);
+    case SQRT: return std::sqrt(x); case CBRT: return std::cbrt(x); case ABS: return std::fabs(x);
+    case CEIL: return std::ceil(x); case FLOOR: return std::floor(x); case TRUNC: return std::trunc(x);
+    case ROUND: return std::round(x);
+  }
+  fail("scalar: bad function");
+  return 0;
+}
+}  // namespace
+
+Variant eval_scalar_node(const Node* n) {
+  auto pv = [&](int i) -> const Variant& {
+    const Node* p = n->prev[i];
+    if (!p->evaluated) fail("scalar: operand not evaluated");
+    return p->value;
+  };
+  const int dt = n->meta.dt;
+  switch (n->kind) {
+    case Kind::Constant: return n->cnst;
+    case Kind::Cast: return variant_convert(pv(0), dt);
+    case Kind::Scalar: return pv(0);
+    case Kind::Conditional: {
+      CV c = to_cv(pv(0));
+      bool t = (c.k == CF32) ? c.f != 0 : (c.k == CF64) ? c.d != 0 : (is_uns(c.k) ? c.u != 0 : c.i != 0);
+      return variant_convert(t ? pv(1) : pv(2), dt);
+    }
+    case Kind::Unary: {
+      const int op = n->op;
+      if (unary_is_operator(op)) {
+        CV a = to_cv(pv(0)), r = a;
+        switch (op) {
+          case POS: break;
+          case NEG:
+            if (a.k == CF32) r.f = -a.f; else if (a.k == CF64) r.d = -a.d;
+            else if (is_uns(a.k)) r.u = (0 - a.u) & (a.k == CU32 ? 0xffffffffull : ~0ull);
+            else r.i = a.k == CI32 ? (int32_t)(0 - (uint64_t)a.i) : (int64_t)(0 - (uint64_t)a.i);
+            break;
+          case NOT: {
+            bool z = a.k == CF32 ? a.f == 0 : a.k == CF64 ? a.d == 0 : is_uns(a.k) ? a.u == 0 : a.i == 0;
+            r = cv_int(z ? 1 : 0);
+            break;
+          }
+          case bNOT:
+            if (is_uns(a.k)) r.u = ~a.u & (a.k == CU32 ? 0xffffffffull : ~0ull);
+            else r.i = a.k == CI32 ? (int32_t)~a.i : ~a.i;
+            break;
+        }
+        return from_cv(r, dt);
+      }
+      if (dt == F32) { Variant v = variant_zero(F32); v.u.f32 = fn1f(op, variant_as<float>(pv(0))); return v; }
+      if (dt == F64) { Variant v = variant_zero(F64); v.u.f64 = fn1(op, variant_as<double>(pv(0))); return v; }
+      if (op == ABS) { CV a = to_cv(pv(0)); if (!is_uns(a.k) && a.i < 0) a.i = -a.i; return from_cv(a, dt); }
+      Variant d = variant_zero(F64); d.u.f64 = fn1(op, variant_as<double>(pv(0)));
+      return variant_convert(d, dt);
+    }
+    case Kind::Binary: {
+      const int op = n->op;
+      if (binary_is_operator(op)) {
+        if (op == MOD && (is_float_dt(pv(0).dtype) || is_float_dt(pv(1).dtype))) {
+          int t = promote(pv(0).dtype, pv(1).dtype);
+          Variant v = variant_zero(t);
+          if (t == F32) v.u.f32 = std::fmod(variant_as<float>(pv(0)), variant_as<float>(pv(1)));
+          else v.u.f64 = std::fmod(variant_as<double>(pv(0)), variant_as<double>(pv(1)));
+          return variant_convert(v, dt);
+        }
+        return from_cv(binop(op, to_cv(pv(0)), to_cv(pv(1))), dt);
+      }
+      Variant l = variant_convert(pv(0), dt), r = variant_convert(pv(1), dt);
+      if (op == MAX2 || op == MIN2) {
+        if (dt == F32) { Variant v = l; v.u.f32 = op == MAX2 ? std::fmax(l.u.f32, r.u.f32) : std::fmin(l.u.f32, r.u.f32); return v; }
+        if (dt == F64) { Variant v = l; v.u.f64 = op == MAX2 ? std::fmax(l.u.f64, r.u.f64) : std::fmin(l.u.f64, r.u.f64); return v; }
+        CV a = to_cv(l), b = to_cv(r);
+        CV gt = binop(GT, a, b);
+        bool lg = gt.i != 0;
+        return (op == MAX2) == lg ? l : r;
+      }
+      if (dt == F32) {
+        float a = l.u.f32, b = r.u.f32, res = 0;
+        switch (op) {
+          case ATAN2: res = std::atan2(a, b); break; case POW: res = std::pow(a, b); break;
+          case HYPOT: res = std::hypot(a, b); break; case FMOD: res = std::fmod(a, b); break;
+        }
+        Variant v = variant_zero(F32); v.u.f32 = res; return v;
+      }
+      double a = variant_as<double>(pv(0)), b = variant_as<double>(pv(1)), res = 0;
+      switch (op) {
+        case ATAN2: res = std::atan2(a, b); break; case POW: res = std::pow(a, b); break;
+        case HYPOT: res = std::hypot(a, b); break; case FMOD: res = std::fmod(a, b); break;
+      }
+      Variant d = variant_zero(F64); d.u.f64 = res;
+      return variant_convert(d, dt);
+    }
+    default: fail(std::string("scalar: cannot evaluate a ") + kind_name(n->kind) + " node on the host");
+  }
+  return Variant{};
+}
+
+// ---- Runtime ---------------------------------------------------------------------
+
+Runtime& Runtime::get() {
+  static Runtime* r = new Runtime();  // never destroyed: device teardown order at exit is the driver's
+  return *r;
+}
+
+static void read_env(Options& o) {
+  if (const char* e = std::getenv("MAPX_FORM")) {
+    std::string s(e);
+    if (s == "plain") o.form = FORM_PLAIN;
+    else if (s == "batched") o.form = FORM_BATCHED;
+  }
+  if (const char* e = std::getenv("MAPX_GRAPH")) o.graph = std::atoi(e) != 0;
+  if (const char* e = std::getenv("MAPX_FUSE_FOCAL_ZONAL")) o.focal_zonal = std::atoi(e) != 0;
+}
+
+void Runtime::setupDevices(const std::string& plat, unsigned dev_type, const std::string& dev_name) {
+  (void)plat; (void)dev_name;
+  if (dev_type != DEV_GPU && dev_type != DEV_DEF && dev_type != DEV_ACC)
+    fail("setupDevices: this backend runs on the GPU only (DEV_GPU)");
+  int device = 0;
+  if (const char* e = std::getenv("MAPX_DEVICE")) device = std::atoi(e);
+  dev::init(device);
+  read_env(opt);
+}
+
+size_t Runtime::rasterBytes(const Meta& m) const {
+  return (size_t)m.nb(0) * m.nb(1) * m.bs.arr[0] * m.bs.arr[1] * sizeof_dt(m.dt);
+}
+
+void* Runtime::residentPtr(Node* n) {
+  auto it = resident_.find(n);
+  if (it == resident_.end()) fail("raster of node " + std::to_string(n->id) + " is not resident");
+  return it->second;
+}
+
+void Runtime::drop(Node* n) {
+  auto it = resident_.find(n);
+  if (it == resident_.end()) return;
+  // a generated raster has no other source: it lives as long as its node
+  if (n->host && n->host->dev == it->second && n->ref > 0) return;
+  store.release(it->second);
+  resident_.erase(it);
+}
+
+static int ext_refs(const Node* n) { return n->ref - (int)n->next.size(); }
+
+void Runtime::forget(Node* n) {
+  auto it = resident_.find(n);
+  if (it == resident_.end()) return;
+  store.release(it->second);
+  resident_.erase(it);
+}
+
+// Host-side stream operations (copies, synchronisation) must not overtake
+// launches that are still being recorded for graph replay: recording is
+// flushed before them and resumed after.
+struct PauseRecording {
+  bool rec;
+  PauseRecording() : rec(dev::recording()) { if (rec) dev::record_flush(); }
+  ~PauseRecording() { if (rec) dev::record_begin(); }
+};
+
+void Runtime::release_unreferenced() {
+  std::vector<Node*> gone;
+  for (auto& r : resident_) if (r.first->ref <= 0) gone.push_back(r.first);
+  for (Node* n : gone) drop(n);
+}
+
+void Runtime::ensure_input(Node* n) {
+  if (resident_.count(n)) return;
+  if (n->kind != Kind::Read) fail("input raster of node " + std::to_string(n->id) + " is not available");
+  const size_t bytes = rasterBytes(n->meta);
+  if (n->host && n->host->dev) fail("generated raster of node " + std::to_string(n->id) + " was released");
+  PauseRecording pause;
+  void* d = store.alloc(bytes);
+  void* pin = dev::malloc_host(bytes);
+  try {
+    if (n->host) std::memcpy(pin, n->host->bytes.data(), bytes);
+    else bin_read_payload(n->path, pin, bytes);
+    dev::h2d(d, pin, bytes);
+    dev::sync();
+  } catch (...) {
+    dev::free_host(pin);
+    store.release(d);
+    throw;
+  }
+  dev::free_host(pin);
+  cnt.bytes_h2d += (long)bytes;
+  resident_[n] = d;
+}
+
+void Runtime::fetch(Node* n, void* host, size_t bytes) {
+  std::lock_guard<std::recursive_mutex> lk(mtx_);
+  if (bytes != rasterBytes(n->meta)) fail("fetch: byte count differs from the raster size");
+  void* d = residentPtr(n);
+  PauseRecording pause;
+  dev::d2h(host, d, bytes);
+  dev::sync();
+  cnt.bytes_d2h += (long)bytes;
+}
+
+void Runtime::write_file(Node* w) {
+  Node* p = w->prev[0];
+  const size_t bytes = rasterBytes(p->meta);
+  std::vector<unsigned char> h(bytes);
+  PauseRecording pause;
+  dev::d2h(h.data(), residentPtr(p), bytes);
+  dev::sync();
+  cnt.bytes_d2h += (long)bytes;
+  bin_write(w->path, p->meta, h.data(), bytes);
+}
+
+std::vector<Node*> Runtime::cut_list(const std::vector<Node*>& targets) {
+  std::set<Node*> seen, tset(targets.begin(), targets.end());
+  std::vector<Node*> stack(targets.begin(), targets.end()), out;
+  while (!stack.empty()) {
+    Node* n = stack.back(); stack.pop_back();
+    if (!seen.insert(n).second) continue;
+    out.push_back(n);
+    if (!tset.count(n)) {
+      bool have = (n->meta.nd == D2 && resident_.count(n)) ||
+                  (n->meta.nd == D0 && n->evaluated && n->kind != Kind::Constant);
+      if (have && n->kind != Kind::Read) { n->cut = true; continue; }
+    }
+    for (Node* p : n->prev) stack.push_back(p);
+  }
+  std::sort(out.begin(), out.end(), [](Node* a, Node* b) {
+    if (a->id != b->id) return a->id < b->id;
+    return (int)a->isOutput() < (int)b->isOutput();
+  });
+  return out;
+}
+
+std::vector<Node*> Runtime::d2_outputs(const Group& g, const std::vector<Node*>& targets) {
+  std::set<Node*> s;
+  for (Node* n : g.outs) {
+    if (n->kind == Kind::Write) s.insert(n->prev[0]);
+    else if (n->kind == Kind::Scalar) continue;
+    else if (n->meta.nd == D2) s.insert(n);
+  }
+  for (Node* t : targets)
+    if (t->meta.nd == D2 && std::find(g.nodes.begin(), g.nodes.end(), t) != g.nodes.end()) s.insert(t);
+  std::vector<Node*> v(s.begin(), s.end());
+  std::sort(v.begin(), v.end(), [](Node* a, Node* b) { return a->id < b->id; });
+  return v;
+}
+
+TileShape Runtime::tile_for(const Group& g) {
+  TileShape t = opt.tile.TX > 0 ? opt.tile : default_tile(g);
+  // a tile never exceeds its block: halve the width (keeping 256 threads by
+  // growing the height) or the height while it does
+  const Meta m = g.meta();
+  if (m.nd == D2) {
+    while (t.TX > m.bs.arr[0] && t.TX / t.VX > 16) { t.TX /= 2; t.TY *= 2; }
+    while (t.TX > m.bs.arr[0] && t.VX > 1) { t.TX /= 2; t.VX /= 2; }
+    while (t.TY > m.bs.arr[1] && t.TY > 256 / (t.TX / t.VX)) t.TY /= 2;
+  }
+  return t;
+}
+
+// Geometry preconditions of the focal template: a halo never exceeds a block
+// nor the valid extent of an edge block, so every halo lookup lands in the
+// central block or one neighbour block.
+static void check_focal_geometry(const Meta& m, int H0, int H1) {
+  const int E0 = m.ds.arr[0] % m.bs.arr[0], E1 = m.ds.arr[1] % m.bs.arr[1];
+  if (H0 > m.bs.arr[0] || H1 > m.bs.arr[1]) fail("focal halo larger than the block size");
+  if ((E0 && E0 < H0) || (E1 && E1 < H1))
+    fail("edge blocks narrower than the focal halo: choose a block size whose remainder is 0 or >= the halo");
+}
+
+void Runtime::scalar_task(const Group& g) {
+  std::vector<Node*> v(g.nodes);
+  v.insert(v.end(), g.outs.begin(), g.outs.end());
+  std::sort(v.begin(), v.end(), [](Node* a, Node* b) {
+    if (a->id != b->id) return a->id < b->id;
+    return (int)a->isOutput() < (int)b->isOutput();
+  });
+  for (Node* n : v) {
+    if (n->meta.nd != D0) continue;
+    if (n->kind == Kind::Constant) continue;
+    n->value = eval_scalar_node(n);
+    n->evaluated = true;
+  }
+  cnt.scalar_tasks++;
+}
+
+namespace {
+template <class T> struct HostIn { const T* p; T v; int fixed; };
+struct HostLaunch { int BS0, BS1, NB0, NB1, DS0, DS1, blk0, nblk, TB0, TB1, part0; };
+
+size_t in_size(int dt) {
+  switch (sizeof_dt(dt)) {
+    case 1: return sizeof(HostIn<uint8_t>); case 2: return sizeof(HostIn<uint16_t>);
+    case 4: return sizeof(HostIn<uint32_t>); default: return sizeof(HostIn<uint64_t>);
+  }
+}
+// Packs an In<T> for a D2 input: pointer, value and fixed flag at their C offsets.
+std::vector<char> pack_in(int dt, const void* p) {
+  std::vector<char> b(in_size(dt), 0);
+  std::memcpy(b.data(), &p, sizeof p);
+  return b;  // value 0, fixed 0: inputs are always materialised rasters
+}
+
+uint32_t morton2(uint32_t x, uint32_t y) {
+  auto spread = [](uint32_t v) {
+    v &= 0xffff;
+    v = (v | (v << 8)) & 0x00ff00ff; v = (v | (v << 4)) & 0x0f0f0f0f;
+    v = (v | (v << 2)) & 0x33333333; v = (v | (v << 1)) & 0x55555555;
+    return v;
+  };
+  return spread(x) | (spread(y) << 1);
+}
+}  // namespace
+
+void Runtime::run_group(const Group& g, const std::vector<Node*>& targets,
+                        const std::vector<std::unique_ptr<Group>>& all) {
+  (void)all;
+  cnt.groups++;
+  if (g.numdim() == D0) { scalar_task(g); return; }
+
+  std::vector<Node*> outs = d2_outputs(g, targets);
+  std::vector<Node*> kouts;
+  for (Node* n : outs)
+    if (std::find(g.nodes.begin(), g.nodes.end(), n) != g.nodes.end() && !resident_.count(n)) kouts.push_back(n);
+  bool has_zonal = false;
+  for (Node* n : g.nodes) if (n->kind == Kind::ZonalReduc || n->kind == Kind::ZonalStats) has_zonal = true;
+
+  for (Node* n : g.ins) {
+    if (n->meta.nd == D2) ensure_input(n);
+    else if (!n->evaluated) fail("D0 input of node " + std::to_string(n->id) + " is not evaluated");
+  }
+
+  if (!kouts.empty() || has_zonal) {
+    const Meta gm = g.meta();
+    for (Node* n : kouts) {
+      if (!dim2_eq(n->meta.ds, gm.ds) || !dim2_eq(n->meta.bs, gm.bs))
+        fail("group outputs differ in raster geometry");
+      void* d = store.alloc(rasterBytes(n->meta));
+      resident_[n] = d;
+    }
+    TileShape tile = tile_for(g);
+    KernelSpec ks = generate_kernel(g, kouts, tile);
+    if (ks.form == Form::Focal) check_focal_geometry(gm, ks.H0, ks.H1);
+    dev::Function fn = dev::function(ks.source, ks.name);
+
+    const int BS0 = gm.bs.arr[0], BS1 = gm.bs.arr[1], NB0 = gm.nb(0), NB1 = gm.nb(1);
+    const int TB0 = (BS0 + tile.TX - 1) / tile.TX, TB1 = (BS1 + tile.TY - 1) / tile.TY;
+    const int gmax = std::max(8, (dev::num_cus() * opt.wg_per_cu) & ~7);
+    auto grid_for = [&](long work) {
+      long gsz = std::min<long>(work, gmax);
+      return (int)((gsz + 7) & ~7L);
+    };
+    std::vector<std::pair<int, int>> launches;  // (blk0, nblk)
+    if (opt.form == FORM_PLAIN) {
+      std::vector<std::pair<uint32_t, int>> order;
+      for (int b = 0; b < NB0 * NB1; b++) order.push_back({morton2(b % NB0, b / NB0), b});
+      std::sort(order.begin(), order.end());
+      for (auto& o : order) launches.push_back({o.second, 1});
+    } else {
+      launches.push_back({0, NB0 * NB1});
+    }
+    long total_parts = 0;
+    for (auto& l : launches) total_parts += grid_for((long)l.second * TB0 * TB1);
+    void* parts = nullptr;
+    void* res = nullptr;
+    if (has_zonal) {
+      parts = store.alloc(std::max<size_t>(1, (size_t)total_parts * ks.part_bytes));
+      res = store.alloc(std::max<size_t>(1, ks.result_bytes));
+    }
+
+    // argument storage in parameter order
+    std::vector<std::vector<char>> store_args;
+    int launch_idx = -1;
+    for (const Param& p : ks.params) {
+      switch (p.kind) {
+        case Param::InD2: store_args.push_back(pack_in(p.dt, resident_.at(p.node))); break;
+        case Param::InD0: {
+          Variant v = variant_convert(p.node->value, p.dt);
+          std::vector<char> b(sizeof_dt(p.dt));
+          std::memcpy(b.data(), &v.u, b.size());
+          store_args.push_back(b);
+          break;
+        }
+        case Param::OutD2: {
+          void* d = resident_.at(p.node);
+          store_args.emplace_back((char*)&d, (char*)&d + sizeof d);
+          break;
+        }
+        case Param::Partials: store_args.emplace_back((char*)&parts, (char*)&parts + sizeof parts); break;
+        case Param::Launch:
+          launch_idx = (int)store_args.size();
+          store_args.emplace_back(sizeof(HostLaunch), 0);
+          break;
+      }
+    }
+    std::vector<void*> ptrs;
+    std::vector<size_t> sizes;
+    for (auto& a : store_args) { ptrs.push_back(a.data()); sizes.push_back(a.size()); }
+    long part0 = 0;
+    for (auto& l : launches) {
+      HostLaunch L{BS0, BS1, NB0, NB1, gm.ds.arr[0], gm.ds.arr[1], l.first, l.second, TB0, TB1, (int)part0};
+      std::memcpy(store_args[launch_idx].data(), &L, sizeof L);
+      const int grid = grid_for((long)l.second * TB0 * TB1);
+      dev::launch(fn, (unsigned)grid, 256, ptrs.data(), sizes.data(), (int)ptrs.size());
+      cnt.launches++;
+      part0 += grid;
+    }
+    if (has_zonal) {
+      dev::Function fin = dev::function(ks.source, ks.fin_name);
+      int np = (int)total_parts;
+      void* fargs[3] = {&parts, &np, &res};
+      size_t fsz[3] = {sizeof parts, sizeof np, sizeof res};
+      dev::launch(fin, 1, 256, fargs, fsz, 3);
+      cnt.launches++;
+      PauseRecording pause;
+      std::vector<unsigned char> host(ks.result_bytes);
+      dev::d2h(host.data(), res, ks.result_bytes);
+      dev::sync();
+      for (const ZonalOut& z : ks.zouts) {
+        if (z.node->kind == Kind::ZonalReduc) {
+          Variant a = variant_zero(z.acc_dt);
+          std::memcpy(&a.u, host.data() + z.result_off, sizeof_dt(z.acc_dt));
+          z.node->value = variant_convert(a, z.node->meta.dt);
+        } else {
+          z.node->table.assign(host.begin() + z.result_off, host.begin() + z.result_off + (size_t)z.nzones * 24);
+        }
+        z.node->evaluated = true;
+      }
+      store.release(parts);
+      store.release(res);
+    }
+  }
+  for (Node* n : g.outs)
+    if (n->kind == Kind::Write) write_file(n);
+}
+
+void Runtime::evaluate(const std::vector<Node*>& targets) {
+  std::lock_guard<std::recursive_mutex> lk(mtx_);
+  if (!dev::initialized()) fail("evaluate: devices not set up (call setupDevices first)");
+  if (targets.empty()) fail("evaluate: nothing to evaluate");
+  release_unreferenced();
+  dag.collect([this](Node* n) { forget(n); });
+  cnt.evals++;
+  dev::Timer t;
+  t.start();
+
+  std::vector<Node*> list = cut_list(targets);
+  struct Uncut {
+    std::vector<Node*>& l;
+    ~Uncut() { for (Node* n : l) n->cut = false; }
+  } uncut{list};
+
+  FuseOptions fo;
+  fo.code_fusion = opt.code_fusion;
+  fo.focal_zonal = opt.focal_zonal;
+  auto groups = Fuser(fo).fuse(list);
+
+  // consumers of each D2 raster in this evaluation, for early release
+  std::map<Node*, int> uses;
+  for (auto& g : groups)
+    for (Node* n : g->ins) if (n->meta.nd == D2) uses[n]++;
+  std::set<Node*> tset(targets.begin(), targets.end());
+  for (Node* t : targets) if (t->kind == Kind::Write) tset.insert(t->prev[0]);
+
+  if (opt.graph) dev::record_begin();
+  try {
+    for (auto& g : groups) {
+      run_group(*g, targets, groups);
+      for (Node* n : g->ins) {
+        if (n->meta.nd != D2) continue;
+        if (--uses[n] == 0 && ext_refs(n) <= 0 && !tset.count(n) && n->kind != Kind::Read) drop(n);
+      }
+    }
+    if (dev::recording()) dev::record_flush();
+  } catch (...) {
+    if (dev::recording()) { try { dev::record_flush(); } catch (...) {} }
+    throw;
+  }
+  // intermediates nobody holds and nothing consumes any more
+  for (auto& g : groups)
+    for (Node* n : g->outs)
+      if (n->meta.nd == D2 && n->kind != Kind::Write && ext_refs(n) <= 0 && !tset.count(n) && uses[n] == 0) drop(n);
+  t.stop();
+  cnt.ms_last_eval = t.ms();
+}
+
+std::string Runtime::plan(const std::vector<Node*>& targets) {
+  std::lock_guard<std::recursive_mutex> lk(mtx_);
+  std::vector<Node*> list = cut_list(targets);
+  struct Uncut {
+    std::vector<Node*>& l;
+    ~Uncut() { for (Node* n : l) n->cut = false; }
+  } uncut{list};
+  FuseOptions fo;
+  fo.code_fusion = opt.code_fusion;
+  fo.focal_zonal = opt.focal_zonal;
+  auto groups = Fuser(fo).fuse(list);
+  auto esc = [](const std::string& s) {
+    std::string o;
+    for (char c : s) {
+      if (c == '"' || c == '\\') { o += '\\'; o += c; }
+      else if (c == '\n') o += "\\n";
+      else if (c == '\t') o += "\\t";
+      else o += c;
+    }
+    return o;
+  };
+  std::ostringstream j;
+  j << "{\"groups\": [";
+  bool fg = true;
+  for (auto& g : groups) {
+    if (!fg) j << ", ";
+    fg = false;
+    j << "{\"id\": " << g->id << ", \"pattern\": " << g->pat << ", \"numdim\": " << g->numdim();
+    auto ids = [&](const char* k, const std::vector<Node*>& v) {
+      j << ", \"" << k << "\": [";
+      for (size_t i = 0; i < v.size(); i++) j << (i ? ", " : "") << v[i]->id;
+      j << "]";
+    };
+    ids("ins", g->ins);
+    ids("nodes", g->nodes);
+    ids("outs", g->outs);
+    j << ", \"kinds\": [";
+    for (size_t i = 0; i < g->nodes.size(); i++) j << (i ? ", " : "") << "\"" << kind_name(g->nodes[i]->kind) << "\"";
+    j << "]";
+    if (g->numdim() != D0) {
+      std::vector<Node*> outs = d2_outputs(*g, targets), kouts;
+      for (Node* n : outs)
+        if (std::find(g->nodes.begin(), g->nodes.end(), n) != g->nodes.end()) kouts.push_back(n);
+      bool has_zonal = false;
+      for (Node* n : g->nodes) if (n->kind == Kind::ZonalReduc || n->kind == Kind::ZonalStats) has_zonal = true;
+      if (!kouts.empty() || has_zonal) {
+        KernelSpec ks = generate_kernel(*g, kouts, tile_for(*g));
+        j << ", \"kernel\": \"" << ks.name << "\", \"form\": " << (int)ks.form << ", \"halo\": [" << ks.H0 << ", " << ks.H1
+          << "], \"source\": \"" << esc(ks.source) << "\"";
+      }
+    }
+    j << "}";
+  }
+  j << "]}";
+  return j.str();
+}
+
+}  // namespace mapx

@@ -1,0 +1,111 @@
+// Block-task execution runtime: evaluation driver, device-resident raster
+// store, per-group kernel launches (plain / batched / graph forms), host
+// scalar tasks and zonal finalisation.
+//
+// Replaces the reference's Runtime::evaluate/workflow (runtime/Runtime.cpp:397-507),
+// Program/Version (kernel generation and compile cache), Scheduler/Worker
+// (job loop) and Cache/Block (device block cache). Rasters stay in HBM whole
+// and block-tiled between fused groups; no host round trip per block.
+#pragma once
+#include "codegen.hpp"
+#include "dag.hpp"
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace mapx {
+
+enum ExecForm : int {
+  FORM_PLAIN = 0,    // one launch per fused group per block (the reference's job granularity)
+  FORM_BATCHED = 1,  // one launch per fused group covering all its blocks
+};
+
+struct Options {
+  int form = FORM_BATCHED;
+  bool graph = false;         // replay identical launch sequences from hipGraphs
+  bool focal_zonal = false;   // fuse FOCAL -> ZONAL into one kernel
+  bool code_fusion = true;
+  int wg_per_cu = 8;          // grid size cap (work-groups per CU)
+  TileShape tile{0, 0, 0};    // forced tile shape (0 = per-group default / tuned)
+};
+
+struct Counters {
+  long evals = 0, groups = 0, launches = 0, scalar_tasks = 0, compiles = 0;
+  long bytes_h2d = 0, bytes_d2h = 0;
+  double ms_last_eval = 0;
+};
+
+// Stream-ordered caching allocator: all device work runs on one in-order
+// stream, so a released buffer can be handed to later work at once.
+class Store {
+ public:
+  void* alloc(size_t bytes);
+  void release(void* p);
+  void trim();        // returns cached buffers to the device
+  size_t in_use() const { return in_use_; }
+  size_t cached() const { return cached_; }
+  ~Store();
+ private:
+  std::multimap<size_t, void*> free_;
+  std::unordered_map<void*, size_t> size_;
+  size_t in_use_ = 0, cached_ = 0;
+};
+
+class Runtime {
+ public:
+  static Runtime& get();
+
+  Dag dag;
+  Options opt;
+  Counters cnt;
+  Store store;
+
+  void setupDevices(const std::string& plat, unsigned dev_type, const std::string& dev_name);
+  void evaluate(const std::vector<Node*>& targets);
+
+  // D2 node whose raster is resident in HBM (after evaluation, while referenced).
+  bool isResident(const Node* n) const { return resident_.count(const_cast<Node*>(n)) != 0; }
+  void* residentPtr(Node* n);
+  size_t rasterBytes(const Meta& m) const;
+  void fetch(Node* n, void* host, size_t bytes);  // copies a resident raster (block-tiled) to host
+  void drop(Node* n);                              // releases a node's resident raster
+  void adopt(Node* n, void* dev) { resident_[n] = dev; }  // raster produced in HBM by a generator
+  void release_unreferenced();
+  void forget(Node* n);  // a node is being destroyed: its raster goes back to the store
+
+  // Plan of an evaluation without executing it (groups and kernel sources), as JSON.
+  std::string plan(const std::vector<Node*>& targets);
+
+ private:
+  struct GroupRun;
+  std::vector<Node*> cut_list(const std::vector<Node*>& targets);
+  void run_group(const Group& g, const std::vector<Node*>& targets, const std::vector<std::unique_ptr<Group>>& all);
+  void scalar_task(const Group& g);
+  void ensure_input(Node* n);
+  std::vector<Node*> d2_outputs(const Group& g, const std::vector<Node*>& targets);
+  TileShape tile_for(const Group& g);
+  void write_file(Node* w);
+
+  std::unordered_map<Node*, void*> resident_;
+  std::recursive_mutex mtx_;
+};
+
+// .bin ROW+BLK file format (file/binary.cpp:221-390): a 4096-byte header of
+// little-endian int32 {data_type, num_dim, ds0, ds1, mem_order, bs0, bs1},
+// zero-padded, then the blocks in proj order (bx + by*NB0), each a full
+// BS0*BS1 row-major block (edge blocks padded).
+struct BinHeader {
+  int dt, nd, ds0, ds1, mo, bs0, bs1;
+};
+BinHeader bin_read_header(const std::string& path);
+void bin_write(const std::string& path, const Meta& m, const void* payload, size_t bytes);
+void bin_read_payload(const std::string& path, void* dst, size_t bytes);
+
+// Host evaluation of a D0 node from its (D0) prevs, with C semantics in the
+// node types (ScalarTask, runtime/task/ScalarTask.cpp:69-193).
+Variant eval_scalar_node(const Node* n);
+
+}  // namespace mapx

@@ -1,0 +1,686 @@
+"""Python 3 front end of the map-algebra API over libmapx.so.
+
+Mirrors the reference's python/map.py (names, enum values, operator
+overloading, constant typing: Python float -> F32 rounded by c_float, int ->
+S32, bool -> B8, python/map.py:111-127) so scripts such as hill.py and
+life.py run unchanged apart from Python 3 syntax. Calls go through ctypes to
+the C ABI declared in include/mapx.h.
+
+Differences from the reference: failures raise MapError instead of aborting
+the process, and symbolic `while` loops over rasters (python/map.py:234-327)
+are not supported (evaluating a Raster as a bool raises).
+Extensions: zstats/zone_table (per-zone statistics), from_array/to_array
+(in-memory rasters), gen_dem/gen_zones (synthetic inputs in HBM),
+set_option/get_option, plan, compile_plan, stats.
+"""
+import builtins as _builtins
+import ctypes as ct
+import json as _json
+import math
+import operator
+import os
+from functools import reduce
+
+import numpy as np
+
+from . import binfile as _binfile
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(_HERE, "libmapx.so")
+
+
+class MapError(RuntimeError):
+    """A failed call into the backend (the reference aborts the process instead)."""
+
+
+def _load():
+    if not os.path.exists(_LIBPATH):
+        raise ImportError("libmapx.so is not built: run `python -m map_amd.build` (%s missing)" % _LIBPATH)
+    return ct.CDLL(_LIBPATH)
+
+
+_lib = _load()
+
+# ---- enumerations (values of python/map.py:35-83) ---------------------------------
+
+DataTypeId = ['NONE_DATATYPE', 'F32', 'F64', 'B8', 'U8', 'U16', 'U32', 'U64', 'S8', 'S16', 'S32', 'S64', 'N_DATATYPE']
+DataTypeVal = list(range(len(DataTypeId)))
+NumDimId = ['NONE_NUMDIM', 'TIME', 'D0', 'D1', 'D2', 'D3', 'N_NUMDIM']
+NumDimVal = [0x00, 0x01, 0x02, 0x04, 0x06, 0x08, 0x0A]
+MemOrderId = ['NONE_MEMORDER', 'BLK', 'ROW', 'COL', 'SFC', 'N_MEMORDER']
+MemOrderVal = [0x00, 0x01, 0x02, 0x04, 0x06, 0x08]
+DeviceTypeId = ['DEV_DEF', 'DEV_CPU', 'DEV_GPU', 'DEV_ACC', 'DEV_CUS', 'DEV_ALL']
+DeviceTypeVal = [0x01, 0x02, 0x04, 0x08, 0x16, 0xFFFFFFFF]
+UnaryTypeId = ['NONE_UNARY', 'POS', 'NEG', 'NOT', 'bNOT', 'MARK_UNARY',
+               'SIN', 'COS', 'TAN', 'ASIN', 'ACOS', 'ATAN', 'SINH', 'COSH', 'TANH', 'ASINH', 'ACOSH', 'ATANH',
+               'EXP', 'EXP2', 'EXP10', 'LOG', 'LOG2', 'LOG10', 'SQRT', 'CBRT', 'ABS',
+               'CEIL', 'FLOOR', 'TRUNC', 'ROUND', 'N_UNARY']
+UnaryTypeSym = ['', '__pos__', '__neg__', '__not__', '__invert__', '',
+                'sin', 'cos', 'tan', 'asin', 'acos', 'atan', 'sinh', 'cosh', 'tanh', 'asinh', 'acosh', 'atanh',
+                'exp', 'exp2', 'exp10', 'log', 'log2', 'log10', 'sqrt', 'cbrt', 'abs',
+                'ceil', 'floor', 'trunc', 'round', '']
+UnaryTypeVal = list(range(len(UnaryTypeId)))
+BinaryTypeId = ['NONE_BINARY', 'ADD', 'SUB', 'MUL', 'DIV', 'MOD', 'EQ', 'NE', 'LT', 'GT', 'LE', 'GE', 'AND', 'OR',
+                'bAND', 'bOR', 'bXOR', 'SHL', 'SHR', 'MARK_BINARY', 'MAX2', 'MIN2', 'ATAN2', 'POW', 'HYPOT', 'FMOD',
+                'N_BINARY']
+BinaryTypeSym = ['', '__add__', '__sub__', '__mul__', '__truediv__', '__mod__',
+                 '__eq__', '__ne__', '__lt__', '__gt__', '__le__', '__ge__', '', '',
+                 '__and__', '__or__', '__xor__', '__lshift__', '__rshift__', '',
+                 'max', 'min', 'atan2', 'pow', 'hypot', 'fmod', '']
+BinaryTypeVal = list(range(len(BinaryTypeId)))
+ReductionTypeId = ['NONE_REDUCTION', 'SUM', 'PROD', 'rAND', 'rOR', 'MARK_REDUCTION', 'MAX', 'MIN', 'N_REDUCTION']
+ReductionTypeVal = list(range(len(ReductionTypeId)))
+
+for _ids, _vals in zip([DataTypeId, NumDimId, MemOrderId, DeviceTypeId, UnaryTypeId, BinaryTypeId, ReductionTypeId],
+                       [DataTypeVal, NumDimVal, MemOrderVal, DeviceTypeVal, UnaryTypeVal, BinaryTypeVal,
+                        ReductionTypeVal]):
+    for _i, _v in zip(_ids, _vals):
+        globals()[_i] = _v
+
+_NP = {F32: np.float32, F64: np.float64, B8: np.uint8, U8: np.uint8, U16: np.uint16, U32: np.uint32,
+       U64: np.uint64, S8: np.int8, S16: np.int16, S32: np.int32, S64: np.int64}
+_CODE = {np.dtype(v): k for k, v in _NP.items() if k != B8}
+
+# ---- by-value ABI structs ------------------------------------------------------------
+
+
+class Variant(ct.Structure):
+    class Union(ct.Union):
+        _fields_ = [('b8', ct.c_bool), ('s8', ct.c_int8), ('s16', ct.c_int16), ('s32', ct.c_int32),
+                    ('s64', ct.c_int64), ('u8', ct.c_uint8), ('u16', ct.c_uint16), ('u32', ct.c_uint32),
+                    ('u64', ct.c_uint64), ('f32', ct.c_float), ('f64', ct.c_double)]
+    _fields_ = [('union', Union), ('dtype', ct.c_int)]
+
+    def __init__(self, val=None, dtype=None):
+        super().__init__()
+        if val is None:
+            return
+        if dtype is not None:
+            self._set(val, dtype)
+        elif isinstance(val, float):
+            self._set(val, F32)
+        elif isinstance(val, bool):
+            self._set(val, B8)
+        elif isinstance(val, int):
+            self._set(val, S32)
+        elif isinstance(val, ct.c_uint8):
+            self._set(val.value, U8)
+        else:
+            raise TypeError("unsupported constant %r" % (val,))
+
+    _FIELD = {F32: 'f32', F64: 'f64', B8: 'b8', U8: 'u8', U16: 'u16', U32: 'u32', U64: 'u64', S8: 's8',
+              S16: 's16', S32: 's32', S64: 's64'}
+
+    def _set(self, val, dt):
+        f = self._FIELD[dt]
+        if dt in (F32, F64):
+            val = float(val)
+        elif dt == B8:
+            val = bool(val)
+        else:
+            val = int(val)
+        setattr(self.union, f, val)
+        self.dtype = dt
+
+    def datatype(self):
+        return self.dtype
+
+    def value(self):
+        if self.dtype not in self._FIELD:
+            raise MapError("empty value")
+        return getattr(self.union, self._FIELD[self.dtype])
+
+    def convert(self, new_dt):
+        return Variant(self.value(), new_dt)
+
+
+class Array(ct.Structure):
+    N = 2
+    T = ct.c_int * N
+    _fields_ = [('arr', T), ('n', ct.c_int)]
+
+    def __init__(self, arg=()):
+        if arg is None or (isinstance(arg, (list, tuple)) and len(arg) == 0):
+            super().__init__(self.T(-1, -1), 0)
+        elif isinstance(arg, (list, tuple)):
+            if len(arg) != self.N:
+                raise ValueError("a size or coordinate needs %d values" % self.N)
+            super().__init__(self.T(*[int(x) for x in arg]), self.N)
+        elif isinstance(arg, Array):
+            super().__init__(arg.arr, arg.n)
+        else:
+            raise TypeError("bad array argument %r" % (arg,))
+
+    def isNone(self):
+        return all(e == -1 for e in self.arr)
+
+    def toList(self):
+        return [x for x in self.arr[:self.n]]
+
+
+class Node(ct.c_void_p):
+    pass
+
+
+def _err(ptr):
+    msg = _lib.ma_lastError().decode()
+    raise MapError(msg or "backend call failed")
+
+
+def _node(ptr):
+    if not ptr:
+        _err(ptr)
+    return ptr
+
+# ---- Raster ------------------------------------------------------------------------
+
+
+class Raster:
+    """Handle of one lazy DAG node (a raster, or a D0 value)."""
+
+    def __init__(self, node=None):
+        if isinstance(node, Node):
+            if not node:
+                _err(node)
+            self._node = node
+            _lib.ma_increaseRef(self)
+        elif node is None:
+            self._node = Node(None)
+        else:
+            raise TypeError("Raster wraps a Node")
+
+    def __del__(self):
+        try:
+            if _lib is not None and self._node:
+                _lib.ma_decreaseRef(self)
+        except Exception:
+            pass
+
+    @classmethod
+    def from_param(cls, ins):
+        return ins._node
+
+    def __bool__(self):
+        raise MapError("symbolic while-loops over rasters are not supported by this backend")
+
+    def __hash__(self):
+        return id(self)
+
+    def __getitem__(self, key):
+        raise MapError("coordinate access is not supported by this backend")
+
+    def __setitem__(self, key, other):
+        if isinstance(key, Raster):  # boolean lhs indexing: self = con(key, other, self)
+            other = _const_to_raster(other).astype(self.datatype())
+            old = self._node
+            self._node = _node(_lib.ma_conditional(key, other, self))
+            _lib.ma_increaseRef(self)
+            _lib.ma_decreaseRef(Node(old))
+            return
+        raise MapError("coordinate assignment is not supported by this backend")
+
+    def __call__(self, ngb):
+        if isinstance(ngb, (list, tuple)) and len(ngb) == 2 and not isinstance(ngb[0], Raster):
+            return Raster(_lib.ma_neighbor(self, Array(list(ngb))))
+        raise MapError("bounded neighbourhoods are not supported by this backend")
+
+    def nodeid(self):
+        return _lib.ma_nodeid(self)
+
+    def datatype(self):
+        return _lib.ma_datatype(self)
+
+    def numdim(self):
+        return _lib.ma_numdim(self)
+
+    def memorder(self):
+        return _lib.ma_memorder(self)
+
+    def datasize(self):
+        return _lib.ma_datasize(self).toList()
+
+    def blocksize(self):
+        return _lib.ma_blocksize(self).toList()
+
+    def numblock(self):
+        ds, bs = self.datasize(), self.blocksize()
+        return [(d + b - 1) // b for d, b in zip(ds, bs)]
+
+    def __pow__(self, other):
+        return Raster(_lib.ma_binary(self, _const_to_raster(other), POW))
+
+    def __rpow__(self, other):
+        return Raster(_lib.ma_binary(_const_to_raster(other), self, POW))
+
+    def no(self):
+        return Raster(_lib.ma_unary(self, NOT))
+
+    def astype(self, dt):
+        return astype(self, dt)
+
+
+def _const_to_raster(arg):
+    if isinstance(arg, Raster):
+        return arg
+    if isinstance(arg, (bool, int, float)):
+        var = Variant(arg)
+        return Raster(_lib.ma_constant(var, Array(), var.datatype(), ROW + BLK, Array()))
+    raise TypeError("cannot use %r as a raster" % (arg,))
+
+
+def _def_unary_op(symbol, utype):
+    def uop(self):
+        return Raster(_lib.ma_unary(self, utype))
+    setattr(Raster, symbol, uop)
+
+
+_SCALAR_FUN = {'exp10': lambda x: 10.0 ** x, 'abs': _builtins.abs, 'round': _builtins.round,
+               'max': _builtins.max, 'min': _builtins.min, 'pow': _builtins.pow}
+
+
+def _scalar_fun(symbol):
+    if symbol in _SCALAR_FUN:
+        return _SCALAR_FUN[symbol]
+    return getattr(math, symbol)
+
+
+def _def_unary_fun(symbol, utype):
+    def ufun(raster):
+        if isinstance(raster, Raster):
+            return Raster(_lib.ma_unary(raster, utype))
+        return _scalar_fun(symbol)(raster)
+    ufun.__name__ = symbol
+    globals()[symbol] = ufun
+
+
+for _sym, _ut in zip(UnaryTypeSym, UnaryTypeVal):
+    if _sym and _ut < MARK_UNARY:
+        _def_unary_op(_sym, _ut)
+    elif _sym and _ut > MARK_UNARY:
+        _def_unary_fun(_sym, _ut)
+
+
+def _def_binary_op(symbol, btype):
+    def biop(self, rhs):
+        return Raster(_lib.ma_binary(self, _const_to_raster(rhs), btype))
+
+    def rbiop(self, lhs):
+        return Raster(_lib.ma_binary(_const_to_raster(lhs), self, btype))
+    setattr(Raster, symbol, biop)
+    setattr(Raster, symbol[:2] + 'r' + symbol[2:], rbiop)
+
+
+def _def_binary_fun(symbol, btype):
+    def bifun(lhs, rhs):
+        if not isinstance(lhs, Raster) and not isinstance(rhs, Raster):
+            return _scalar_fun(symbol)(lhs, rhs)
+        return Raster(_lib.ma_binary(_const_to_raster(lhs), _const_to_raster(rhs), btype))
+    bifun.__name__ = symbol
+    globals()[symbol] = bifun
+
+
+for _sym, _bt in zip(BinaryTypeSym, BinaryTypeVal):
+    if _sym and _bt < MARK_BINARY:
+        _def_binary_op(_sym, _bt)
+    elif _sym and _bt > MARK_BINARY:
+        _def_binary_fun(_sym, _bt)
+Raster.__div__ = Raster.__truediv__
+
+# ---- module functions (python/map.py:451-576) -------------------------------------------
+
+
+def setupDevices(plat_name, dev_type, dev_name):
+    _lib.ma_setupDevices(plat_name.encode(), dev_type, dev_name.encode())
+    if _lib.ma_lastError():
+        _err(None)
+
+
+def eval(*args):
+    cond = [isinstance(a, Raster) for a in args]
+    if len(args) and not all(cond):
+        if not any(cond):
+            return _builtins.eval(*args)
+        raise TypeError("eval takes only rasters, or only the arguments of the built-in eval")
+    arr = (Node * len(args))(*[r._node for r in args])
+    _lib.ma_eval(ct.cast(arr, ct.POINTER(Node)), len(args))
+    if _lib.ma_lastError():
+        _err(None)
+
+
+def value(arg):
+    var = _lib.ma_value(arg)
+    if var.dtype == 0:
+        _err(None)
+    return var.value()
+
+
+def read(file):
+    return Raster(_lib.ma_read(file.encode() if isinstance(file, str) else file))
+
+
+def write(raster, file):
+    r = _lib.ma_write(raster, file.encode() if isinstance(file, str) else file)
+    if r != 0:
+        _err(None)
+    return r
+
+
+def _cnst(cnst, ds, dt, mo, bs):
+    var = Variant(cnst).convert(dt)
+    return Raster(_lib.ma_constant(var, Array(ds), dt, mo, Array(bs)))
+
+
+def zeros(ds=(), dt=F32, mo=ROW + BLK, bs=()):
+    return _cnst(0, ds, dt, mo, bs)
+
+
+def ones(ds=(), dt=F32, mo=ROW + BLK, bs=()):
+    return _cnst(1, ds, dt, mo, bs)
+
+
+def trues(ds=(), dt=B8, mo=ROW + BLK, bs=()):
+    return _cnst(True, ds, dt, mo, bs)
+
+
+def falses(ds=(), dt=B8, mo=ROW + BLK, bs=()):
+    return _cnst(False, ds, dt, mo, bs)
+
+
+def full(cnst, ds=(), dt=F32, mo=ROW + BLK, bs=()):
+    return _cnst(cnst, ds, dt, mo, bs)
+
+
+def _like(cnst, raster, dt, mo):
+    dt = raster.datatype() if dt == NONE_DATATYPE else dt
+    mo = raster.memorder() if mo == NONE_MEMORDER else mo
+    var = Variant(cnst).convert(dt)
+    return Raster(_lib.ma_constant(var, Array(raster.datasize()), dt, mo, Array(raster.blocksize())))
+
+
+def zeros_like(raster, dt=NONE_DATATYPE, mo=NONE_MEMORDER):
+    return _like(0, raster, dt, mo)
+
+
+def ones_like(raster, dt=NONE_DATATYPE, mo=NONE_MEMORDER):
+    return _like(1, raster, dt, mo)
+
+
+def full_like(cnst, raster, dt=NONE_DATATYPE, mo=NONE_MEMORDER):
+    return _like(cnst, raster, dt, mo)
+
+
+def index(raster, dim):
+    return Raster(_lib.ma_index(raster, dim))
+
+
+def astype(raster, dt):
+    return Raster(_lib.ma_cast(raster, dt))
+
+
+def con(cond, lhs, rhs):
+    return Raster(_lib.ma_conditional(_const_to_raster(cond), _const_to_raster(lhs), _const_to_raster(rhs)))
+
+
+def _mask(lst):
+    if not lst or not all(len(e) == len(lst[0]) for e in lst):
+        raise ValueError("a mask is a non-empty list of equal-length rows")
+    ds = Array([len(lst[0]), len(lst)])
+    flat = [e for row in lst for e in row]
+    arr = (Variant * len(flat))(*[Variant(e) for e in flat])
+    return ct.cast(arr, ct.POINTER(Variant)), len(flat), ds, arr
+
+
+def convolve(raster, lst):
+    ptr, n, ds, keep = _mask(lst)
+    return Raster(_lib.ma_convolution(raster, ptr, n, ds))
+
+
+def fsum(raster):
+    return convolve(raster, [[1, 1, 1], [1, 1, 1], [1, 1, 1]])
+
+
+def _focal(raster, lst, red):
+    ptr, n, ds, keep = _mask(lst)
+    return Raster(_lib.ma_focalFunc(raster, ptr, n, ds, red))
+
+
+def fmin(raster, lst):
+    return _focal(raster, lst, MIN)
+
+
+def fmax(raster, lst):
+    return _focal(raster, lst, MAX)
+
+
+def zsum(raster):
+    return Raster(_lib.ma_zonalReduc(raster, SUM))
+
+
+def zprod(raster):
+    return Raster(_lib.ma_zonalReduc(raster, PROD))
+
+
+def zmax(raster):
+    return Raster(_lib.ma_zonalReduc(raster, MAX))
+
+
+def zmin(raster):
+    return Raster(_lib.ma_zonalReduc(raster, MIN))
+
+
+def lmax(lst):
+    return reduce(lambda a, b: max(a, b), lst)
+
+
+def lmin(lst):
+    return reduce(lambda a, b: min(a, b), lst)
+
+
+def lor(lst):
+    return reduce(lambda a, b: a + b, lst)
+
+
+def pick(raster, lst):
+    acu = _const_to_raster(lst[0])
+    for i in range(len(lst)):
+        acu = con(raster == i, lst[i], acu)
+    return acu
+
+
+def degrees(raster):
+    return raster / (2 * math.pi) * 360
+
+
+def radians(raster):
+    return raster / 360 * (2 * math.pi)
+
+
+def mean(raster):
+    return zsum(raster) / prod(raster.datasize())
+
+
+def gradient(raster):
+    ix = convolve(raster, [[-1, 0, 1], [-1, 0, 1], [-1, 0, 1]])
+    iy = convolve(raster, [[-1, -1, -1], [0, 0, 0], [1, 1, 1]])
+    return ix, iy
+
+
+def blur(raster, size=3):
+    if size == 3:
+        return convolve(raster, [[1, 2, 1], [2, 4, 2], [1, 2, 1]]) / 16
+    if size == 5:
+        k = [[1, 4, 7, 4, 1], [4, 16, 26, 16, 4], [7, 26, 41, 26, 7], [4, 16, 26, 16, 4], [1, 4, 7, 4, 1]]
+        return convolve(raster, k) / 276
+    raise ValueError("blur size must be 3 or 5")
+
+
+def harris(raster):
+    ix, iy = gradient(raster)
+    xx, yy, xy = blur(ix * ix), blur(iy * iy), blur(ix * iy)
+    return (xx * yy - xy) - 0.04 * (xx + yy) ** 2
+
+
+def prod(arg):
+    if isinstance(arg, (list, tuple)):
+        return reduce(operator.mul, arg, 1)
+    if isinstance(arg, Array):
+        return arg.arr[0] * arg.arr[1]
+    raise TypeError("prod of a list or Array")
+
+# ---- extensions ------------------------------------------------------------------------
+
+
+class ZoneStats(Raster):
+    """Per-zone statistics node: after eval, table() gives sum/min/max/count/mean."""
+
+    def __init__(self, node, nzones):
+        super().__init__(node)
+        self.nzones = nzones
+
+    def table(self):
+        n = self.nzones
+        s, mn, mx = (np.zeros(n, np.float64) for _ in range(3))
+        c = np.zeros(n, np.int64)
+        dp = ct.POINTER(ct.c_double)
+        r = _lib.ma_zonalTable(self, s.ctypes.data_as(dp), mn.ctypes.data_as(dp), mx.ctypes.data_as(dp),
+                               c.ctypes.data_as(ct.POINTER(ct.c_longlong)), n)
+        if r != 0:
+            _err(None)
+        with np.errstate(invalid='ignore', divide='ignore'):
+            mean_ = np.where(c > 0, s / np.maximum(c, 1), np.nan)
+        return {'sum': s, 'sum_f32': s.astype(np.float32), 'min': mn, 'max': mx, 'count': c, 'mean': mean_}
+
+
+def zstats(raster, zones, nzones):
+    return ZoneStats(_lib.ma_zonalStats(raster, zones, int(nzones)), int(nzones))
+
+
+def zone_table(zs):
+    return zs.table()
+
+
+def from_array(a, bs, dt=None):
+    """In-memory input raster from a 2D numpy array (rows = y)."""
+    a = np.ascontiguousarray(a)
+    code = dt if dt is not None else _CODE[a.dtype]
+    flat = _binfile.to_blocks(a.astype(_NP[code], copy=False), bs)
+    return Raster(_lib.ma_input(flat.ctypes.data_as(ct.c_void_p), ct.c_size_t(flat.nbytes),
+                                Array([a.shape[1], a.shape[0]]), code, Array(list(bs))))
+
+
+def to_array(raster):
+    """Evaluated raster -> 2D numpy array of its data size (evaluates it if needed)."""
+    ds, bs, dt = raster.datasize(), raster.blocksize(), raster.datatype()
+    nb = raster.numblock()
+    flat = np.empty(nb[0] * nb[1] * bs[0] * bs[1], dtype=_NP[dt])
+    if _lib.ma_fetch(raster, flat.ctypes.data_as(ct.c_void_p), ct.c_size_t(flat.nbytes)) != 0:
+        eval(raster)
+        if _lib.ma_fetch(raster, flat.ctypes.data_as(ct.c_void_p), ct.c_size_t(flat.nbytes)) != 0:
+            _err(None)
+    return _binfile.from_blocks(flat, ds, bs)
+
+
+def release(raster):
+    if _lib.ma_release(raster) != 0:
+        _err(None)
+
+
+def gen_dem(ds, bs, gauss, noise_seed):
+    """Synthetic DEM in HBM: `gauss` is an (n, 4) array of (cx, cy, sigma, amplitude)."""
+    g = np.ascontiguousarray(np.asarray(gauss, dtype=np.float64).reshape(-1, 4))
+    return Raster(_lib.ma_genDem(Array(list(ds)), Array(list(bs)), g.ctypes.data_as(ct.POINTER(ct.c_double)),
+                                 len(g), ct.c_ulonglong(noise_seed)))
+
+
+def gen_zones(ds, bs, px, py, dt=S32):
+    px = np.ascontiguousarray(px, dtype=np.int32)
+    py = np.ascontiguousarray(py, dtype=np.int32)
+    ip = ct.POINTER(ct.c_int)
+    return Raster(_lib.ma_genZones(Array(list(ds)), Array(list(bs)), dt, px.ctypes.data_as(ip),
+                                   py.ctypes.data_as(ip), len(px)))
+
+
+def set_option(key, val):
+    if _lib.ma_setOption(key.encode(), int(val)) != 0:
+        _err(None)
+
+
+def get_option(key):
+    return _lib.ma_getOption(key.encode())
+
+
+def _nodes(args):
+    arr = (Node * len(args))(*[r._node for r in args])
+    return ct.cast(arr, ct.POINTER(Node)), len(args), arr
+
+
+def plan(*args):
+    ptr, n, keep = _nodes(args)
+    s = _lib.ma_plan(ptr, n)
+    if not s:
+        _err(None)
+    return _json.loads(s.decode())
+
+
+def compile_plan(*args):
+    ptr, n, keep = _nodes(args)
+    r = _lib.ma_compilePlan(ptr, n)
+    if r < 0:
+        _err(None)
+    return r
+
+
+def stats():
+    return _json.loads(_lib.ma_stats().decode())
+
+
+def last_eval_ms():
+    return _lib.ma_lastEvalMs()
+
+# ---- ctypes prototypes (python/map.py:723-830) ------------------------------------------
+
+
+def _proto(name, args, res):
+    f = getattr(_lib, name)
+    f.argtypes = args
+    f.restype = res
+
+
+_proto('ma_lastError', [], ct.c_char_p)
+_proto('ma_setupDevices', [ct.c_char_p, ct.c_int, ct.c_char_p], None)
+_proto('ma_increaseRef', [Raster], None)
+_proto('ma_decreaseRef', [Raster], None)
+_proto('ma_eval', [ct.POINTER(Node), ct.c_int], None)
+_proto('ma_nodeid', [Raster], ct.c_int)
+_proto('ma_value', [Raster], Variant)
+_proto('ma_datatype', [Raster], ct.c_int)
+_proto('ma_numdim', [Raster], ct.c_int)
+_proto('ma_memorder', [Raster], ct.c_int)
+_proto('ma_datasize', [Raster], Array)
+_proto('ma_blocksize', [Raster], Array)
+_proto('ma_read', [ct.c_char_p], Node)
+_proto('ma_write', [Raster, ct.c_char_p], ct.c_int)
+_proto('ma_index', [Raster, ct.c_int], Node)
+_proto('ma_cast', [Raster, ct.c_int], Node)
+_proto('ma_conditional', [Raster, Raster, Raster], Node)
+_proto('ma_unary', [Raster, ct.c_int], Node)
+_proto('ma_binary', [Raster, Raster, ct.c_int], Node)
+_proto('ma_constant', [Variant, Array, ct.c_int, ct.c_int, Array], Node)
+_proto('ma_neighbor', [Raster, Array], Node)
+_proto('ma_convolution', [Raster, ct.POINTER(Variant), ct.c_int, Array], Node)
+_proto('ma_focalFunc', [Raster, ct.POINTER(Variant), ct.c_int, Array, ct.c_int], Node)
+_proto('ma_zonalReduc', [Raster, ct.c_int], Node)
+_proto('ma_zonalStats', [Raster, Raster, ct.c_int], Node)
+_proto('ma_zonalTable', [Raster, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double), ct.POINTER(ct.c_double),
+                         ct.POINTER(ct.c_longlong), ct.c_int], ct.c_int)
+_proto('ma_input', [ct.c_void_p, ct.c_size_t, Array, ct.c_int, Array], Node)
+_proto('ma_fetch', [Raster, ct.c_void_p, ct.c_size_t], ct.c_int)
+_proto('ma_release', [Raster], ct.c_int)
+_proto('ma_setOption', [ct.c_char_p, ct.c_longlong], ct.c_int)
+_proto('ma_getOption', [ct.c_char_p], ct.c_longlong)
+_proto('ma_plan', [ct.POINTER(Node), ct.c_int], ct.c_char_p)
+_proto('ma_compilePlan', [ct.POINTER(Node), ct.c_int], ct.c_int)
+_proto('ma_stats', [], ct.c_char_p)
+_proto('ma_lastEvalMs', [], ct.c_double)
+_proto('ma_genDem', [Array, Array, ct.POINTER(ct.c_double), ct.c_int, ct.c_ulonglong], Node)
+_proto('ma_genZones', [Array, Array, ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), ct.c_int], Node)

@@ -1,0 +1,557 @@
+"""CPU oracle evaluator of map-algebra scripts. TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+bench.py as the checker; the product path never imports it.
+
+It mirrors the API of map_amd.map (convolve, atan, sqrt, atan2, sin, cos,
+con, zsum, ... and the Raster operators) but evaluates eagerly on whole
+numpy rasters with the reference's semantics:
+
+* node types: util/promote.hpp:22-157 (restated independently below),
+  relational/logical ops give U8 (runtime/dag/Binary.cpp:53-57,
+  Unary.cpp:40-41), convolution type promote(input, mask type)
+  (Convolution.cpp:37), mask type = type of the first weight (util/Mask.cpp:18);
+* each statement in C arithmetic: operands converted to C's common type,
+  the result converted to the node type (runtime/skeleton/Skeleton.cpp:179-257);
+* python float constants are F32 (rounded by c_float), ints S32
+  (python/map.py:111-127); float mask weights pass through "%f"
+  (util/VariantType.cpp:91, skeleton/util.cpp:136-165);
+* focal ops with the block-relative border rule (oracle.c, load_F of
+  skeleton/util.cpp:492-506).
+
+Two modes: "f32" (reference semantics: FP32 nodes rounded to FP32 after every
+statement; transcendental functions evaluated in FP64 and rounded once, i.e.
+a correctly rounded FP32 libm) and "f64" (every floating node computed in
+FP64 from the same FP32 inputs and constants: the exact-arithmetic value of
+the script, the FP64 oracle).
+"""
+import ctypes as ct
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            from . import build as _b
+            _b.build()
+        _LIB = ct.CDLL(path)
+        i64p = ct.POINTER(ct.c_int64)
+        _LIB.or_tap_index.argtypes = [ct.c_int] * 6 + [i64p]
+        _LIB.or_conv_f32.argtypes = [ct.c_void_p, i64p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_void_p]
+        _LIB.or_conv_f64.argtypes = [ct.c_void_p, i64p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_void_p]
+        _LIB.or_conv_i64.argtypes = [ct.c_void_p, i64p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_void_p]
+        _LIB.or_focal_f64.argtypes = [ct.c_void_p, i64p, ct.c_int64, ct.c_int, ct.c_int, ct.c_void_p]
+        _LIB.or_life.argtypes = [ct.c_void_p, ct.c_void_p] + [ct.c_int] * 5
+        _LIB.or_zstats_f32.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_void_p,
+                                       ct.c_void_p, ct.c_void_p]
+        _LIB.or_dem.argtypes = [ct.c_void_p, ct.c_int64, ct.c_int64, ct.c_int, ct.c_int, ct.c_void_p, ct.c_int,
+                                ct.c_uint64]
+        _LIB.or_voronoi.argtypes = [ct.c_void_p, ct.c_int64, ct.c_int64, ct.c_int, ct.c_int, ct.c_void_p,
+                                    ct.c_void_p, ct.c_int]
+        _LIB.or_src.argtypes = [ct.c_int] * 8 + [ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]
+    return _LIB
+
+
+# type codes (python/map.py:35-36)
+F32, F64, B8, U8, U16, U32, U64, S8, S16, S32, S64 = range(1, 12)
+SUM, PROD, rAND, rOR, MAX, MIN = 1, 2, 3, 4, 6, 7
+NP = {F32: np.float32, F64: np.float64, B8: np.uint8, U8: np.uint8, U16: np.uint16, U32: np.uint32,
+      U64: np.uint64, S8: np.int8, S16: np.int16, S32: np.int32, S64: np.int64}
+CODE = {np.dtype(np.float32): F32, np.dtype(np.float64): F64, np.dtype(np.uint8): U8, np.dtype(np.uint16): U16,
+        np.dtype(np.uint32): U32, np.dtype(np.uint64): U64, np.dtype(np.int8): S8, np.dtype(np.int16): S16,
+        np.dtype(np.int32): S32, np.dtype(np.int64): S64}
+FLOATS = (F32, F64)
+UNSIGNED = (B8, U8, U16, U32, U64)
+
+_MODE = ["f32"]
+
+
+def set_mode(m):
+    assert m in ("f32", "f64")
+    _MODE[0] = m
+
+
+def mode():
+    return _MODE[0]
+
+
+# util/promote.hpp:22-157 as a rank table: floats dominate; otherwise the
+# table's rules (restated pairwise, including its irregular entries).
+
+_PROMOTE = {
+    F32: [F32, F64, F32, F32, F32, F32, F32, F32, F32, F32, F32],
+    F64: [F64] * 11,
+    B8: [F32, F64, B8, U8, U16, U32, U64, S8, S16, S32, S64],
+    U8: [F32, F64, U8, U8, U16, U32, U64, S8, S16, S32, S64],
+    U16: [F32, F64, U16, U16, U16, U32, U64, U16, S16, S32, S64],
+    U32: [F32, F64, U32, U32, U32, U32, U64, S32, S32, S32, S64],
+    U64: [F32, F64, B8, U64, U64, U64, U64, S64, S64, S64, S64],
+    S8: [F32, F64, S8, S8, S16, S32, S64, S8, S16, S32, S64],
+    S16: [F32, F64, S16, S16, S16, S32, S64, S16, S16, S32, S64],
+    S32: [F32, F64, S32, S32, S32, S32, S64, S32, S32, S32, S64],
+    S64: [F32, F64, S64, S64, S64, S64, S64, S64, S64, S64, S64],
+}
+
+
+def promote(a, b):
+    return _PROMOTE[a][b - 1]
+
+
+# C's integer promotion and usual arithmetic conversions (C11 6.3.1.1, 6.3.1.8)
+def _c_promoted(dt):
+    if dt in (B8, U8, S8, U16, S16, S32):
+        return np.int32
+    return {U32: np.uint32, S64: np.int64, U64: np.uint64, F32: np.float32, F64: np.float64}[dt]
+
+
+def _c_common(a, b):
+    pa, pb = np.dtype(_c_promoted(a)), np.dtype(_c_promoted(b))
+    if np.float64 in (pa, pb):
+        return np.float64
+    if np.float32 in (pa, pb):
+        return np.float32
+    if pa == pb:
+        return pa.type
+    ua, ub = pa.kind == 'u', pb.kind == 'u'
+    if ua == ub:
+        return (pa if pa.itemsize >= pb.itemsize else pb).type
+    u, s = (pa, pb) if ua else (pb, pa)
+    if u.itemsize >= s.itemsize:
+        return u.type
+    return s.type
+
+
+def _work(np_t):
+    """Storage type of a C type in the current mode (FP64 mode widens floats)."""
+    if _MODE[0] == "f64" and np.dtype(np_t).kind == 'f':
+        return np.float64
+    return np_t
+
+
+def _conv(v, dt):
+    """C conversion of values v to the node type dt."""
+    t = _work(NP[dt])
+    v = np.asarray(v)
+    if dt == B8:
+        return (v != 0).astype(np.uint8)
+    if np.dtype(t).kind in 'iu' and v.dtype.kind == 'f':
+        with np.errstate(invalid='ignore'):
+            return np.trunc(v).astype(np.int64).astype(t)
+    with np.errstate(over='ignore'):
+        return v.astype(t)
+
+
+class R:
+    """An oracle raster (2D, rows = y) or D0 value with its reference type."""
+
+    def __init__(self, val, dt, bs=None):
+        self.val = val
+        self.dt = dt
+        self.bs = bs
+
+    @property
+    def d2(self):
+        return self.bs is not None
+
+    def datasize(self):
+        return [self.val.shape[1], self.val.shape[0]]
+
+    def blocksize(self):
+        return list(self.bs)
+
+    def datatype(self):
+        return self.dt
+
+    def __bool__(self):
+        raise TypeError("oracle rasters are not booleans")
+
+    def __hash__(self):
+        return id(self)
+
+    def __pow__(self, o):
+        return binary(self, o, 'POW')
+
+    def __rpow__(self, o):
+        return binary(o, self, 'POW')
+
+    def astype(self, dt):
+        return astype(self, dt)
+
+    def __neg__(self):
+        return unary(self, 'NEG')
+
+    def __pos__(self):
+        return unary(self, 'POS')
+
+    def __invert__(self):
+        return unary(self, 'bNOT')
+
+    def no(self):
+        return unary(self, 'NOT')
+
+
+for _op, _name in [('ADD', 'add'), ('SUB', 'sub'), ('MUL', 'mul'), ('DIV', 'truediv'), ('MOD', 'mod'),
+                   ('EQ', 'eq'), ('NE', 'ne'), ('LT', 'lt'), ('GT', 'gt'), ('LE', 'le'), ('GE', 'ge'),
+                   ('bAND', 'and'), ('bOR', 'or'), ('bXOR', 'xor'), ('SHL', 'lshift'), ('SHR', 'rshift')]:
+    def _mk(op):
+        return (lambda self, o: binary(self, o, op)), (lambda self, o: binary(o, self, op))
+    _f, _rf = _mk(_op)
+    setattr(R, '__%s__' % _name, _f)
+    setattr(R, '__r%s__' % _name, _rf)
+
+
+def const(v):
+    if isinstance(v, R):
+        return v
+    if isinstance(v, float):
+        return R(_work(np.float32)(np.float32(v)), F32)
+    if isinstance(v, bool):
+        return R(np.uint8(v), B8)
+    if isinstance(v, int):
+        return R(np.int32(v), S32)
+    raise TypeError(v)
+
+
+def from_array(a, bs):
+    a = np.asarray(a)
+    dt = CODE[a.dtype]
+    return R(a.astype(_work(a.dtype.type)), dt, tuple(bs))
+
+
+def _shape_of(*xs):
+    bs = None
+    for x in xs:
+        if x.d2:
+            bs = x.bs
+    return bs
+
+
+def _cdiv(a, b):
+    q = a // b
+    adj = (a % b != 0) & ((a < 0) ^ (b < 0))
+    return q + adj.astype(q.dtype)
+
+
+def binary(l, r, op):
+    l, r = const(l), const(r)
+    bs = _shape_of(l, r)
+    rel = op in ('EQ', 'NE', 'LT', 'GT', 'LE', 'GE', 'AND', 'OR')
+    dt = U8 if rel else promote(l.dt, r.dt)
+    if op in ('MAX2', 'MIN2', 'ATAN2', 'POW', 'HYPOT', 'FMOD'):
+        a, b = _conv(l.val, dt), _conv(r.val, dt)
+        if dt in FLOATS:
+            a64, b64 = a.astype(np.float64), b.astype(np.float64)
+            f = {'MAX2': np.fmax, 'MIN2': np.fmin, 'ATAN2': np.arctan2, 'POW': np.power, 'HYPOT': np.hypot,
+                 'FMOD': np.fmod}[op]
+            with np.errstate(all='ignore'):
+                res = f(a64, b64)
+            return R(_conv(res, dt), dt, bs)
+        if op in ('MAX2', 'MIN2'):
+            return R((np.maximum if op == 'MAX2' else np.minimum)(a, b), dt, bs)
+        f = {'ATAN2': np.arctan2, 'POW': np.power, 'HYPOT': np.hypot, 'FMOD': np.fmod}[op]
+        return R(_conv(f(a.astype(np.float64), b.astype(np.float64)), dt), dt, bs)
+    ct_ = _work(_c_common(l.dt, r.dt))
+    a = np.asarray(l.val).astype(ct_)
+    b = np.asarray(r.val).astype(ct_)
+    with np.errstate(all='ignore'):
+        if op == 'ADD':
+            res = a + b
+        elif op == 'SUB':
+            res = a - b
+        elif op == 'MUL':
+            res = a * b
+        elif op == 'DIV':
+            res = a / b if np.dtype(ct_).kind == 'f' else _cdiv(a, b)
+        elif op == 'MOD':
+            res = np.fmod(a, b) if np.dtype(ct_).kind == 'f' else a - b * _cdiv(a, b)
+        elif op in ('EQ', 'NE', 'LT', 'GT', 'LE', 'GE'):
+            res = {'EQ': a == b, 'NE': a != b, 'LT': a < b, 'GT': a > b, 'LE': a <= b, 'GE': a >= b}[op]
+        elif op == 'AND':
+            res = (a != 0) & (b != 0)
+        elif op == 'OR':
+            res = (a != 0) | (b != 0)
+        elif op == 'bAND':
+            res = a & b
+        elif op == 'bOR':
+            res = a | b
+        elif op == 'bXOR':
+            res = a ^ b
+        elif op == 'SHL':
+            res = a << b
+        elif op == 'SHR':
+            res = a >> b
+        else:
+            raise ValueError(op)
+    if _MODE[0] == "f32" and np.dtype(ct_) == np.float32:
+        res = np.asarray(res).astype(np.float32)
+    return R(_conv(res, dt), dt, bs)
+
+
+_UFUN = {'SIN': np.sin, 'COS': np.cos, 'TAN': np.tan, 'ASIN': np.arcsin, 'ACOS': np.arccos, 'ATAN': np.arctan,
+         'SINH': np.sinh, 'COSH': np.cosh, 'TANH': np.tanh, 'ASINH': np.arcsinh, 'ACOSH': np.arccosh,
+         'ATANH': np.arctanh, 'EXP': np.exp, 'EXP2': np.exp2, 'EXP10': lambda x: np.power(10.0, x),
+         'LOG': np.log, 'LOG2': np.log2, 'LOG10': np.log10, 'SQRT': np.sqrt, 'CBRT': np.cbrt, 'ABS': np.abs,
+         'CEIL': np.ceil, 'FLOOR': np.floor, 'TRUNC': np.trunc, 'ROUND': lambda x: np.trunc(x + np.copysign(0.5, x))}
+
+
+def unary(x, op):
+    x = const(x)
+    dt = U8 if op == 'NOT' else x.dt
+    if op in ('POS', 'NEG', 'bNOT'):
+        p = np.asarray(x.val).astype(_work(_c_promoted(x.dt)))
+        with np.errstate(over='ignore'):
+            res = p if op == 'POS' else (-p if op == 'NEG' else ~p)
+        return R(_conv(res, dt), dt, x.bs)
+    if op == 'NOT':
+        return R((np.asarray(x.val) == 0).astype(np.uint8), U8, x.bs)
+    if op == 'ABS' and dt not in FLOATS:
+        p = np.asarray(x.val).astype(_c_promoted(x.dt))
+        return R(_conv(np.abs(p), dt), dt, x.bs)
+    with np.errstate(all='ignore'):
+        res = _UFUN[op](np.asarray(x.val).astype(np.float64))
+    return R(_conv(res, dt), dt, x.bs)
+
+
+def astype(x, dt):
+    return R(_conv(x.val, dt), dt, x.bs)
+
+
+def con(c, l, r):
+    c, l, r = const(c), const(l), const(r)
+    if l.dt != r.dt:
+        raise TypeError("conditional branches differ in type")
+    bs = _shape_of(c, l, r)
+    return R(_conv(np.where(np.asarray(c.val) != 0, l.val, r.val), l.dt), l.dt, bs)
+
+
+def _quantize(w):
+    """A mask weight as the reference's kernel source spells it (std::to_string, %f)."""
+    if isinstance(w, float):
+        return float(np.float32(float("%f" % float(np.float32(w)))))
+    return w
+
+
+_TAPCACHE = {}
+
+
+def taps(ds, bs, h0, h1):
+    key = (tuple(ds), tuple(bs), h0, h1)
+    if key not in _TAPCACHE:
+        nt = (2 * h0 + 1) * (2 * h1 + 1)
+        idx = np.empty(ds[0] * ds[1] * nt, dtype=np.int64)
+        lib().or_tap_index(ds[0], ds[1], bs[0], bs[1], h0, h1, idx.ctypes.data_as(ct.POINTER(ct.c_int64)))
+        if len(_TAPCACHE) > 8:
+            _TAPCACHE.clear()
+        _TAPCACHE[key] = (idx, nt)
+    return _TAPCACHE[key]
+
+
+def convolve(x, lst):
+    if not x.d2:
+        raise TypeError("convolve needs a raster")
+    ncol, nrow = len(lst[0]), len(lst)
+    flat = [e for row in lst for e in row]
+    first = flat[0]
+    mdt = F32 if isinstance(first, float) else (B8 if isinstance(first, bool) else S32)
+    dt = promote(x.dt, mdt)
+    h0, h1 = ncol // 2, nrow // 2
+    ds = x.datasize()
+    idx, nt = taps(ds, x.bs, h0, h1)
+    n = ds[0] * ds[1]
+    L = lib()
+    i64p = ct.POINTER(ct.c_int64)
+    if dt in FLOATS:
+        if _MODE[0] == "f64" or dt == F64:
+            w = np.array([float(_quantize(e)) if mdt == F32 else float(e) for e in flat], np.float64)
+            src = np.ascontiguousarray(np.asarray(x.val, np.float64).reshape(-1))
+            out = np.empty(n, np.float64)
+            L.or_conv_f64(src.ctypes.data, idx.ctypes.data_as(i64p), n, nt, w.ctypes.data, out.ctypes.data)
+        else:
+            w = np.array([_quantize(e) if mdt == F32 else e for e in flat], np.float32)
+            src = np.ascontiguousarray(np.asarray(x.val).astype(np.float32).reshape(-1))
+            out = np.empty(n, np.float32)
+            L.or_conv_f32(src.ctypes.data, idx.ctypes.data_as(i64p), n, nt, w.ctypes.data, out.ctypes.data)
+        return R(_conv(out.reshape(ds[1], ds[0]), dt), dt, x.bs)
+    w = np.array([int(e) for e in flat], np.int64)
+    src = np.ascontiguousarray(np.asarray(x.val).astype(np.int64).reshape(-1))
+    out = np.empty(n, np.int64)
+    L.or_conv_i64(src.ctypes.data, idx.ctypes.data_as(i64p), n, nt, w.ctypes.data, out.ctypes.data)
+    return R(_conv(out.reshape(ds[1], ds[0]), dt), dt, x.bs)
+
+
+def neighbor(x, coord):
+    dx, dy = coord
+    h0, h1 = abs(dx), abs(dy)
+    ds = x.datasize()
+    idx, nt = taps(ds, x.bs, h0, h1)
+    t = (dy + h1) * (2 * h0 + 1) + (dx + h0)
+    src = np.asarray(x.val).reshape(-1)
+    return R(src[idx.reshape(-1, nt)[:, t]].reshape(ds[1], ds[0]), x.dt, x.bs)
+
+
+def _focal(x, lst, red):
+    ncol, nrow = len(lst[0]), len(lst)
+    h0, h1 = ncol // 2, nrow // 2
+    ds = x.datasize()
+    idx, nt = taps(ds, x.bs, h0, h1)
+    src = np.asarray(x.val).reshape(-1)
+    g = src[idx.reshape(-1, nt)]
+    if red == MAX:
+        res = g.max(axis=1)
+    elif red == MIN:
+        res = g.min(axis=1)
+    else:
+        raise NotImplementedError
+    return R(res.reshape(ds[1], ds[0]).astype(src.dtype), x.dt, x.bs)
+
+
+def fmin(x, lst):
+    return _focal(x, lst, MIN)
+
+
+def fmax(x, lst):
+    return _focal(x, lst, MAX)
+
+
+def fsum(x):
+    return convolve(x, [[1, 1, 1], [1, 1, 1], [1, 1, 1]])
+
+
+def _zred(x, red):
+    v = np.asarray(x.val)
+    dt = x.dt
+    if red in (MAX, MIN):
+        r = v.max() if red == MAX else v.min()
+        return R(_conv(r, dt), dt)
+    if dt in FLOATS:
+        a = v.astype(np.float64).reshape(-1)
+        r = math.fsum(a) if red == SUM else float(np.prod(a))
+        return R(_conv(np.float64(r), dt), dt)
+    a = v.astype(np.int64).reshape(-1)
+    if red == SUM:
+        r = int(a.sum(dtype=np.int64))
+    elif red == PROD:
+        r = 1
+        for e in a:
+            r = (r * int(e)) & ((1 << 64) - 1)
+    elif red == rAND:
+        r = int(np.bitwise_and.reduce(a))
+    else:
+        r = int(np.bitwise_or.reduce(a))
+    bits = np.dtype(NP[dt]).itemsize * 8
+    r &= (1 << bits) - 1
+    return R(np.array(r, dtype=np.uint64).astype(NP[dt]), dt)
+
+
+def zsum(x):
+    return _zred(x, SUM)
+
+
+def zprod(x):
+    return _zred(x, PROD)
+
+
+def zmax(x):
+    return _zred(x, MAX)
+
+
+def zmin(x):
+    return _zred(x, MIN)
+
+
+def zstats(x, zones, nz):
+    """Per-zone (sum FP64, min, max, count, mean) of x over integer zones."""
+    v = np.asarray(x.val).astype(np.float64).reshape(-1)
+    z = np.asarray(zones.val).astype(np.int64).reshape(-1)
+    ok = (z >= 0) & (z < nz)
+    v, z = v[ok], z[ok]
+    cnt = np.bincount(z, minlength=nz).astype(np.int64)
+    s = np.zeros(nz, np.float64)
+    # exact-ish FP64 sums per zone (sorted by zone, fsum per group)
+    order = np.argsort(z, kind='stable')
+    zs, vs = z[order], v[order]
+    bounds = np.searchsorted(zs, np.arange(nz + 1))
+    mn = np.full(nz, np.inf)
+    mx = np.full(nz, -np.inf)
+    for k in range(nz):
+        seg = vs[bounds[k]:bounds[k + 1]]
+        if len(seg):
+            s[k] = math.fsum(seg)
+            mn[k] = seg.min()
+            mx[k] = seg.max()
+    with np.errstate(invalid='ignore', divide='ignore'):
+        mean = np.where(cnt > 0, s / np.maximum(cnt, 1), np.nan)
+    return {'sum': s, 'min': mn, 'max': mx, 'count': cnt, 'mean': mean}
+
+
+def value(x):
+    return np.asarray(x.val).item()
+
+
+def _def_u(name, op):
+    def f(x):
+        if isinstance(x, R):
+            return unary(x, op)
+        return getattr(math, name)(x)
+    f.__name__ = name
+    globals()[name] = f
+
+
+for _name, _op in [('sin', 'SIN'), ('cos', 'COS'), ('tan', 'TAN'), ('asin', 'ASIN'), ('acos', 'ACOS'),
+                   ('atan', 'ATAN'), ('sinh', 'SINH'), ('cosh', 'COSH'), ('tanh', 'TANH'), ('exp', 'EXP'),
+                   ('exp2', 'EXP2'), ('log', 'LOG'), ('log2', 'LOG2'), ('log10', 'LOG10'), ('sqrt', 'SQRT'),
+                   ('cbrt', 'CBRT'), ('ceil', 'CEIL'), ('floor', 'FLOOR'), ('trunc', 'TRUNC')]:
+    _def_u(_name, _op)
+
+
+def _def_b(name, op, scalar):
+    def f(a, b):
+        if not isinstance(a, R) and not isinstance(b, R):
+            return scalar(a, b)
+        return binary(a, b, op)
+    f.__name__ = name
+    globals()[name] = f
+
+
+_def_b('atan2', 'ATAN2', math.atan2)
+_def_b('hypot', 'HYPOT', math.hypot)
+_def_b('fmod', 'FMOD', math.fmod)
+_def_b('max', 'MAX2', max)
+_def_b('min', 'MIN2', min)
+
+
+def life(state_u8, bs, gens):
+    """`gens` Life generations of a U8 raster (rows = y) with the reference semantics."""
+    a = np.ascontiguousarray(state_u8, dtype=np.uint8).copy()
+    b = np.empty_like(a)
+    lib().or_life(a.ctypes.data, b.ctypes.data, a.shape[1], a.shape[0], bs[0], bs[1], gens)
+    return a if gens % 2 == 0 else b
+
+
+def dem(x0, y0, w, h, gauss, seed):
+    g = np.ascontiguousarray(np.asarray(gauss, np.float64).reshape(-1, 4))
+    out = np.empty((h, w), np.float32)
+    lib().or_dem(out.ctypes.data, x0, y0, w, h, g.ctypes.data, len(g), ct.c_uint64(seed))
+    return out
+
+
+def voronoi(x0, y0, w, h, px, py):
+    px = np.ascontiguousarray(px, np.int32)
+    py = np.ascontiguousarray(py, np.int32)
+    out = np.empty((h, w), np.int32)
+    lib().or_voronoi(out.ctypes.data, x0, y0, w, h, px.ctypes.data, py.ctypes.data, len(px))
+    return out
+
+
+def src_cell(ds, bs, bx, by, lx, ly):
+    gx, gy = ct.c_int(), ct.c_int()
+    lib().or_src(ds[0], ds[1], bs[0], bs[1], bx, by, lx, ly, ct.byref(gx), ct.byref(gy))
+    return gx.value, gy.value

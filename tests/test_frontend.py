@@ -1,0 +1,103 @@
+"""Front end, DAG typing, value numbering, fusion grouping and kernel
+generation (compiled with hiprtc for gfx950; no GPU needed)."""
+import numpy as np
+import pytest
+
+from tests import scripts
+
+
+def dem(M, w=64, h=48, bs=(32, 16)):
+    return M.from_array(np.zeros((h, w), np.float32), bs)
+
+
+def test_constant_typing(M):
+    r = dem(M)
+    assert (r * 2).datatype() == M.F32
+    assert (r * 2.5).datatype() == M.F32
+    u = M.from_array(np.zeros((4, 4), np.uint8), (4, 4))
+    assert (u + 1).datatype() == M.S32          # promote(U8, S32)
+    assert (u == 1).datatype() == M.U8           # relational -> U8
+    assert (u * u).datatype() == M.U8
+    assert M.convolve(u, scripts.LIFE_MASK).datatype() == M.S32
+    assert M.convolve(r, scripts.LIFE_MASK).datatype() == M.F32
+    assert M.zsum(u).numdim() == M.D0 and M.zsum(u).datatype() == M.U8
+    assert M.sqrt(r).datatype() == M.F32
+    assert u.no().datatype() == M.U8
+
+
+def test_value_numbering(M):
+    r = dem(M)
+    a = M.convolve(r, [[1, 2, 1], [0, 0, 0], [-1, -2, -1]])
+    b = M.convolve(r, [[1, 2, 1], [0, 0, 0], [-1, -2, -1]])
+    assert a.nodeid() == b.nodeid()
+    assert (a + 1).nodeid() == (a + 1).nodeid()
+    assert (a + 1).nodeid() != (a + 1.0).nodeid()
+
+
+def test_errors_raise(M):
+    a = dem(M, 64, 48)
+    b = dem(M, 32, 48)
+    with pytest.raises(M.MapError):
+        a + b
+    with pytest.raises(M.MapError):
+        M.value(M.zsum(a))  # not evaluated
+    with pytest.raises(M.MapError):
+        M.convolve(a, [[1, 1], [1, 1]])  # even mask
+
+
+def test_hillshade_is_one_fused_group(M):
+    p = M.plan(scripts.hillshade(M, dem(M)))
+    assert len(p["groups"]) == 1
+    g = p["groups"][0]
+    assert g["pattern"] & 0x04 and g["pattern"] & 0x02  # FOCAL and LOCAL
+    assert g["kinds"].count("Convolution") == 2
+    assert g["halo"] == [1, 1]
+    # D0 constants enter as kernel arguments (Fusioner.cpp:494-508)
+    assert len(g["ins"]) == 10
+
+
+def test_life_generations_share_one_kernel(M):
+    s = M.from_array(np.zeros((64, 64), np.uint8), (32, 32))
+    p = M.plan(scripts.life(M, s, 6))
+    assert len(p["groups"]) == 6
+    assert len({g["kernel"] for g in p["groups"]}) == 1
+
+
+def test_scalar_chain_is_a_host_group(M):
+    r = dem(M)
+    mean = M.zsum(r) / 3072
+    p = M.plan(mean)
+    kinds = [g["kinds"] for g in p["groups"]]
+    assert ["ZonalReduc"] in kinds and ["Binary"] in kinds
+    assert any(g["numdim"] == M.D0 for g in p["groups"])
+
+
+def test_focal_zonal_fusion_is_opt_in(M):
+    r = dem(M)
+    z = M.zsum(scripts.hillshade(M, r))
+    assert len(M.plan(z)["groups"]) == 2  # FOCAL -> ZONAL not fused (Pattern.cpp:87)
+    M.set_option("focal_zonal", 1)
+    try:
+        assert len(M.plan(z)["groups"]) == 1
+    finally:
+        M.set_option("focal_zonal", 0)
+
+
+def test_kernels_compile_for_gfx950(M):
+    r = dem(M)
+    u = M.from_array(np.zeros((64, 64), np.uint8), (32, 32))
+    zones = M.from_array(np.zeros((48, 64), np.int32), (32, 16))
+    exprs = [scripts.hillshade(M, r), scripts.life(M, u, 2), M.zsum(r), M.zmax(u), M.zmin(r * 2),
+             M.zstats(r, zones, 16), M.fmax(r, [[1, 1, 1], [1, 1, 1], [1, 1, 1]]), r(  [1, -1]) + r([-1, 0]),
+             M.con(r > 0, r, -r), M.astype(r, M.S32) % 7]
+    for e in exprs:
+        assert M.compile_plan(e) >= 1
+
+
+def test_tile_fits_the_block(M):
+    import re
+    for side, bs in [(96, 48), (1024, 1024), (64, 16)]:
+        u = M.from_array(np.zeros((side, side), np.uint8), (bs, bs))
+        src = M.plan(scripts.life(M, u, 1))["groups"][0]["source"]
+        tx, ty, vx = map(int, re.search(r"focal_body<P, (\d+), (\d+), (\d+)>", src).groups())
+        assert tx <= bs and ty <= bs and 256 % (tx // vx) == 0, (bs, tx, ty, vx)

@@ -1,0 +1,161 @@
+"""GPU parity of the HIP path against the CPU oracle (same seeded inputs).
+
+Integer and boolean results are compared bit for bit; FP32 results against
+the FP64 oracle with the ULP / cancellation bound of tests/parity.py, and
+against the FP32-semantics oracle for reference.
+"""
+import numpy as np
+import pytest
+
+from tests import scripts
+from tests.parity import assert_exact, assert_fp32, fp32_report
+
+pytestmark = pytest.mark.gpu
+
+
+def dem_array(w, h, seed):
+    from oracle import ref as O
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = 16
+    g = np.stack([rng.uniform(0, w, n), rng.uniform(0, h, n), rng.uniform(8, 80, n), rng.uniform(0, 3000, n)], 1)
+    return O.dem(0, 0, w, h, g, seed)
+
+
+def _ulps(got, ref):
+    got = np.asarray(got, np.float32).astype(np.float64)
+    ref = np.asarray(ref, np.float32).astype(np.float64)
+    sp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    return np.abs(got - ref) / sp
+
+
+@pytest.mark.parametrize("w,h,bs", [(256, 192, (64, 64)), (300, 220, (128, 64)), (1024, 1024, (256, 256))])
+def test_hillshade(G, O, w, h, bs):
+    """Hillshade through the HIP path vs the reference-semantics FP32 oracle.
+
+    Exact FP32 stages (Sobel sums in reference order, /8, x*x+y*y, sqrt) are
+    bit-exact; every transcendental node (atan, atan2, sin, cos) is within 4
+    ULP of the correctly rounded FP32 result on the same FP32 input; the
+    composed output's error against the FP64 oracle is reported for both the
+    GPU and the FP32-semantics oracle.
+    """
+    print("case hillshade", w, h, bs, flush=True)
+    dem = dem_array(w, h, 7)
+    D = G.from_array(dem, bs)
+    od = O.from_array(dem, bs)
+    # exact stages: bit-exact
+    r_g = G.to_array(G.sqrt(scripts.hori(G, D) * scripts.hori(G, D) + scripts.vert(G, D) * scripts.vert(G, D)))
+    x_o, y_o = scripts.hori(O, od), scripts.vert(O, od)
+    r_o = O.sqrt(x_o * x_o + y_o * y_o)
+    assert_exact(r_g, r_o.val, "sobel + sqrt")
+    # transcendental nodes on identical FP32 inputs: <= 4 ULP of correctly rounded
+    R_in = G.from_array(r_o.val, bs)
+    X_in, Y_in = G.from_array(x_o.val, bs), G.from_array(y_o.val, bs)
+    s_o = O.atan(r_o)
+    S_in = G.from_array(s_o.val, bs)
+    asp_o = scripts.aspect(O, od)
+    arg_o = O.const(float(np.float32((360 - 315.0 + 90) / 180 * scripts.PI))) - asp_o
+    A_in = G.from_array(arg_o.val, bs)
+    for name, g, o in [("atan", G.atan(R_in), s_o), ("atan2", G.atan2(Y_in, -X_in), O.atan2(y_o, -x_o)),
+                       ("sin", G.sin(S_in), O.sin(s_o)), ("cos", G.cos(S_in), O.cos(s_o)),
+                       ("cos(ar-aspect)", G.cos(A_in), O.cos(arg_o))]:
+        u = _ulps(G.to_array(g), o.val)
+        print("  %s: max %.2f ulp, mean %.4f" % (name, u.max(), u.mean()), flush=True)
+        assert u.max() <= 4.0, "%s: %.2f ulp" % (name, u.max())
+    # composed output: FP64 error reported for the GPU and for the FP32-semantics oracle
+    got = G.to_array(scripts.hillshade(G, D))
+    O.set_mode("f64")
+    t1, t2 = scripts.hillshade_terms(O, O.from_array(dem, bs))
+    ref64 = (t1 + t2).val
+    scale = np.abs(t1.val) + np.abs(t2.val)
+    O.set_mode("f32")
+    ref32 = scripts.hillshade(O, od).val
+    print("  gpu vs fp64:", fp32_report(got, ref64, scale), flush=True)
+    print("  fp32 oracle vs fp64:", fp32_report(ref32, ref64, scale), flush=True)
+    print("  gpu vs fp32 oracle: max %.1f ulp, max abs %.3g" % (_ulps(got, ref32).max(),
+                                                                np.abs(got.astype(np.float64) - ref32).max()), flush=True)
+    assert np.all(np.isfinite(got))
+    assert np.abs(got.astype(np.float64) - ref64).max() <= 2 * np.abs(ref32.astype(np.float64) - ref64).max() + 1e-6
+
+
+@pytest.mark.parametrize("w,h,bs,gens", [(200, 150, (64, 32), 12), (512, 512, (128, 128), 40), (1024, 768, (256, 256), 25)])
+def test_life(G, O, w, h, bs, gens):
+    print("case life", w, h, bs, gens, flush=True)
+    rng = np.random.Generator(np.random.PCG64(1))
+    s0 = (rng.random((h, w)) < 0.3).astype(np.uint8)
+    got = G.to_array(scripts.life(G, G.from_array(s0, bs), gens))
+    ref = O.life(s0, bs, gens)
+    assert_exact(got, ref, "life")
+
+
+def test_life_forms_identical(G, O):
+    """Plain (one launch per block), batched and graph-replayed forms agree bit for bit."""
+    rng = np.random.Generator(np.random.PCG64(2))
+    s0 = (rng.random((384, 512)) < 0.3).astype(np.uint8)
+    bs = (128, 128)
+    outs = []
+    try:
+        for form, graph in [(0, 0), (1, 0), (1, 1), (1, 1)]:
+            G.set_option("form", form)
+            G.set_option("graph", graph)
+            outs.append(G.to_array(scripts.life(G, G.from_array(s0, bs), 10)))
+    finally:
+        G.set_option("form", 1)
+        G.set_option("graph", 0)
+    ref = O.life(s0, bs, 10)
+    for o in outs:
+        assert_exact(o, ref, "life forms")
+
+
+def test_local_integer_ops(G, O):
+    rng = np.random.Generator(np.random.PCG64(3))
+    a = rng.integers(0, 256, (100, 130)).astype(np.uint8)
+    b = rng.integers(-1000, 1000, (100, 130)).astype(np.int32)
+    bs = (32, 64)
+
+    def script(M, A, B):
+        c = A * 3 + B
+        d = (c > 100) * (A - 7) + (c <= 100) * B / 3
+        e = M.con(A < 128, A * A, A + 1)
+        f = (B % 7) + (-A) + (A == 5)
+        return d + e * 2 + f
+    got = G.to_array(script(G, G.from_array(a, bs), G.from_array(b, bs)))
+    ref = script(O, O.from_array(a, bs), O.from_array(b, bs))
+    assert_exact(got, ref.val, "integer local ops")
+
+
+def test_zonal_global(G, O):
+    rng = np.random.Generator(np.random.PCG64(4))
+    a = rng.integers(0, 256, (300, 200)).astype(np.uint8)
+    b = rng.integers(-5000, 5000, (300, 200)).astype(np.int32)
+    f = rng.normal(0, 100, (300, 200)).astype(np.float32)
+    bs = (64, 64)
+    A, B, F = G.from_array(a, bs), G.from_array(b, bs), G.from_array(f, bs)
+    rs = [G.zsum(A), G.zsum(B), G.zmax(B), G.zmin(B), G.zmax(F), G.zmin(F), G.zsum(F), G.zsum(G.astype(A, G.S32))]
+    G.eval(*rs)
+    vals = [G.value(r) for r in rs]
+    oa, ob, of = O.from_array(a, bs), O.from_array(b, bs), O.from_array(f, bs)
+    want = [O.value(O.zsum(oa)), O.value(O.zsum(ob)), O.value(O.zmax(ob)), O.value(O.zmin(ob)),
+            O.value(O.zmax(of)), O.value(O.zmin(of))]
+    assert vals[:6] == want, (vals, want)
+    s64 = float(np.sum(f.astype(np.float64)))
+    bound = 4 * np.finfo(np.float32).eps * float(np.sum(np.abs(f.astype(np.float64))))
+    assert abs(vals[6] - s64) <= bound, (vals[6], s64)
+    assert vals[7] == int(a.astype(np.int64).sum())
+
+
+def test_zstats(G, O):
+    rng = np.random.Generator(np.random.PCG64(5))
+    w, h, nz = 520, 390, 37
+    px, py = rng.integers(0, w, nz), rng.integers(0, h, nz)
+    zones = O.voronoi(0, 0, w, h, px, py)
+    f = rng.normal(10, 50, (h, w)).astype(np.float32)
+    bs = (128, 128)
+    zs = G.zstats(G.from_array(f, bs), G.from_array(zones, bs), nz)
+    G.eval(zs)
+    t = zs.table()
+    ref = O.zstats(O.from_array(f, bs), O.from_array(zones, bs), nz)
+    assert np.array_equal(t["count"], ref["count"])
+    assert np.array_equal(t["min"], ref["min"])
+    assert np.array_equal(t["max"], ref["max"])
+    tol = 1e-12 * np.maximum(1.0, np.abs(ref["sum"])) * 64
+    assert np.all(np.abs(t["sum"] - ref["sum"]) <= tol), np.max(np.abs(t["sum"] - ref["sum"]))

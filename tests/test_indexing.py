@@ -1,0 +1,110 @@
+"""Host-side check of the focal template's global indexing.
+
+Emulates, for small raster geometries, every global read the focal kernel
+body makes (device/mapx_dev.hiph: tiles of each block, halo cells inside
+[-H, E+H), the block-relative border rule of ldf, clamped per-cell loads)
+and checks that each lands inside the raster allocation and inside the block
+it claims. Geometries include tiles wider than blocks and partial edge
+blocks (the cases that would read out of bounds without the halo bound).
+"""
+import itertools
+
+import pytest
+
+
+def invert(i, d):
+    return -i - 1 if i < 0 else (d - (i - d + 1) if i >= d else i)
+
+
+def modulo(i, d):
+    return i + d if i < 0 else (i - d if i >= d else i)
+
+
+def ldf_index(ds, bs, nb, bx, by, x, y):
+    e0 = min(bs[0], ds[0] - bx * bs[0])
+    e1 = min(bs[1], ds[1] - by * bs[1])
+    i0 = 0 if x < 0 else (2 if x >= e0 else 1)
+    i1 = 0 if y < 0 else (2 if y >= e1 else 1)
+    blk = by * nb[0] + bx
+    if i0 != 1 or i1 != 1:
+        nbx, nby = bx + i0 - 1, by + i1 - 1
+        if nbx < 0 or nby < 0 or nbx >= nb[0] or nby >= nb[1]:
+            x, y = invert(x, e0), invert(y, e1)
+        else:
+            x, y = modulo(x, bs[0]), modulo(y, bs[1])
+            blk = nby * nb[0] + nbx
+    return blk, x, y
+
+
+def tile_for(bs, tx, ty, vx):
+    """Mirror of Runtime::tile_for's clamping."""
+    while tx > bs[0] and tx // vx > 16:
+        tx //= 2
+        ty *= 2
+    while tx > bs[0] and vx > 1:
+        tx //= 2
+        vx //= 2
+    while ty > bs[1] and ty > 256 // (tx // vx):
+        ty //= 2
+    return tx, ty, vx
+
+
+GEOMS = [((512, 384), (128, 128), 1, 1, (256, 16, 4)),
+         ((200, 150), (64, 32), 1, 1, (256, 16, 4)),
+         ((300, 220), (128, 64), 1, 1, (64, 16, 1)),
+         ((70, 41), (32, 16), 2, 2, (64, 16, 1)),
+         ((33, 17), (16, 16), 1, 1, (256, 16, 4)),
+         ((40, 40), (40, 40), 3, 3, (64, 16, 1))]
+
+
+@pytest.mark.parametrize("ds,bs,h0,h1,tile", GEOMS)
+def test_focal_reads_in_bounds(ds, bs, h0, h1, tile):
+    tx, ty, vx = tile_for(bs, *tile)
+    twx = tx // vx
+    assert 256 % twx == 0 and ty % (256 // twx) == 0
+    nb = ((ds[0] + bs[0] - 1) // bs[0], (ds[1] + bs[1] - 1) // bs[1])
+    total = nb[0] * nb[1] * bs[0] * bs[1]
+    tb = ((bs[0] + tx - 1) // tx, (bs[1] + ty - 1) // ty)
+    hw, hh = tx + 2 * h0, ty + 2 * h1
+    for b in range(nb[0] * nb[1]):
+        bx, by = b % nb[0], b // nb[0]
+        e0 = min(bs[0], ds[0] - bx * bs[0])
+        e1 = min(bs[1], ds[1] - by * bs[1])
+        assert e0 >= h0 and e1 >= h1
+        for t in range(tb[0] * tb[1]):
+            x0, y0 = (t % tb[0]) * tx, (t // tb[0]) * ty
+            if x0 >= e0 or y0 >= e1:
+                continue
+            for ex, ey in itertools.product(range(hw), range(hh)):
+                x, y = x0 - h0 + ex, y0 - h1 + ey
+                if not (x < e0 + h0 and y < e1 + h1):
+                    continue
+                blk, sx, sy = ldf_index(ds, bs, nb, bx, by, x, y)
+                assert 0 <= sx < bs[0] and 0 <= sy < bs[1], (b, t, x, y, sx, sy)
+                be0 = min(bs[0], ds[0] - (blk % nb[0]) * bs[0])
+                be1 = min(bs[1], ds[1] - (blk // nb[0]) * bs[1])
+                assert sx < be0 and sy < be1, "read of padding at %s" % ((b, x, y),)
+                idx = (blk * bs[1] + sy) * bs[0] + sx
+                assert 0 <= idx < total
+            # per-cell loads use clamped coordinates
+            for lx, ly in itertools.product(range(tx), range(ty)):
+                x, y = x0 + lx, y0 + ly
+                cx, cy = (x, y) if (x < e0 and y < e1) else (e0 - 1, e1 - 1)
+                assert 0 <= (b * bs[1] + cy) * bs[0] + cx < total
+
+
+def test_border_rule_matches_oracle():
+    """The emulated rule and the oracle's C restatement agree on every lookup."""
+    from oracle import ref as O
+    for ds, bs in [((70, 41), (32, 16)), ((64, 64), (16, 16)), ((9, 7), (4, 4))]:
+        nb = ((ds[0] + bs[0] - 1) // bs[0], (ds[1] + bs[1] - 1) // bs[1])
+        for b in range(nb[0] * nb[1]):
+            bx, by = b % nb[0], b // nb[0]
+            e0 = min(bs[0], ds[0] - bx * bs[0])
+            e1 = min(bs[1], ds[1] - by * bs[1])
+            for x in range(-1, e0 + 1):
+                for y in range(-1, e1 + 1):
+                    blk, sx, sy = ldf_index(ds, bs, nb, bx, by, x, y)
+                    gx = (blk % nb[0]) * bs[0] + sx
+                    gy = (blk // nb[0]) * bs[1] + sy
+                    assert (gx, gy) == O.src_cell(ds, bs, bx, by, x, y)
