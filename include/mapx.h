@@ -124,7 +124,7 @@ int ma_release(ma_node_t *node);
 /* Runtime option: "form" (0 plain: one launch per group per block, 1 batched),
  * "graph" (replay launch sequences from hipGraphs), "focal_zonal" (fuse a focal
  * group into the zonal group that consumes it), "tile_tx"/"tile_ty"/"tile_vx",
- * "wg_per_cu". Returns 0 on success. */
+ * "wg_per_cu", "profile" (time every launch). Returns 0 on success. */
 int ma_setOption(const char *key, long long value);
 long long ma_getOption(const char *key);
 /* Fusion groups and generated kernel sources of an evaluation, as JSON, without
@@ -141,6 +141,9 @@ ma_node_t *ma_genDem(ma_dim2_t ds, ma_dim2_t bs, const double *gauss, int ng, un
 ma_node_t *ma_genZones(ma_dim2_t ds, ma_dim2_t bs, ma_datatype_t dt, const int *px, const int *py, int np);
 /* Device time of the last evaluation in milliseconds (HIP events). */
 double ma_lastEvalMs(void);
+/* Durations of the kernels launched since the last call while the "profile"
+ * option is set (HIP events around each launch), as JSON by kernel name. */
+const char *ma_kernelTimes(void);
 /* Fusion legality of a pattern pair (PatternEnum bit sets): bit 0 pipe, bit 1
  * flat (runtime/Pattern.cpp:53-182 as applied by the fuser). */
 int ma_fusionRules(int top_or_left, int bot_or_right);

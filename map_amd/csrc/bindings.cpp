@@ -222,6 +222,7 @@ int ma_setOption(const char* key, long long value) {
     } else if (k == "graph") o.graph = value != 0;
     else if (k == "focal_zonal") o.focal_zonal = value != 0;
     else if (k == "code_fusion") o.code_fusion = value != 0;
+    else if (k == "profile") dev::profile_set(value != 0);
     else if (k == "tile_tx") o.tile.TX = (int)value;
     else if (k == "tile_ty") o.tile.TY = (int)value;
     else if (k == "tile_vx") o.tile.VX = (int)value;
@@ -240,6 +241,7 @@ long long ma_getOption(const char* key) {
   if (k == "graph") return o.graph;
   if (k == "focal_zonal") return o.focal_zonal;
   if (k == "code_fusion") return o.code_fusion;
+  if (k == "profile") return dev::profile_on();
   if (k == "tile_tx") return o.tile.TX;
   if (k == "tile_ty") return o.tile.TY;
   if (k == "tile_vx") return o.tile.VX;
@@ -289,12 +291,29 @@ const char* ma_stats(void) {
     << ", \"scalar_tasks\": " << r.cnt.scalar_tasks << ", \"bytes_h2d\": " << r.cnt.bytes_h2d
     << ", \"bytes_d2h\": " << r.cnt.bytes_d2h << ", \"store_in_use\": " << r.store.in_use()
     << ", \"store_cached\": " << r.store.cached() << ", \"graphs\": " << dev::graph_cache_size()
-    << ", \"ms_last_eval\": " << r.cnt.ms_last_eval << "}";
+    << ", \"ms_last_eval\": " << r.cnt.ms_last_eval << ", \"ms_fuse\": " << r.cnt.ms_fuse
+    << ", \"ms_codegen\": " << r.cnt.ms_codegen << ", \"ms_launch\": " << r.cnt.ms_launch << "}";
   g_json = s.str();
   return g_json.c_str();
 }
 
 double ma_lastEvalMs(void) { return R().cnt.ms_last_eval; }
+
+const char* ma_kernelTimes(void) {
+  return guard([&] {
+    std::ostringstream s;
+    s << "{";
+    bool first = true;
+    for (auto& k : dev::profile_take()) {
+      s << (first ? "" : ", ") << "\"" << k.name << "\": {\"count\": " << k.count << ", \"total_ms\": " << k.total_ms
+        << ", \"min_ms\": " << k.min_ms << ", \"max_ms\": " << k.max_ms << "}";
+      first = false;
+    }
+    s << "}";
+    g_json = s.str();
+    return g_json.c_str();
+  }, (const char*)nullptr);
+}
 
 int ma_fusionRules(int top_or_left, int bot_or_right) {
   return guard([&] {

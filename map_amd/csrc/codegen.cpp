@@ -104,9 +104,10 @@ TileShape default_tile(const Group& g) {
       minsz = std::min(minsz, sizeof_dt(n->meta.dt));
   if (minsz == 8) minsz = 4;
   TileShape t;
-  if (minsz == 1) { t.VX = 4; t.TX = 256; t.TY = 16; }
-  else if (minsz == 2) { t.VX = 2; t.TX = 128; t.TY = 16; }
-  else { t.VX = 1; t.TX = 64; t.TY = 16; }
+  // every type moves VX = 4 cells per thread (16 bytes for FP32); 64 threads
+  // per tile row, so a wavefront covers 256 contiguous cells of a row
+  if (minsz >= 8) { t.VX = 2; t.TX = 128; t.TY = 16; }
+  else { t.VX = 4; t.TX = 256; t.TY = 16; }
   return t;
 }
 
@@ -233,10 +234,12 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
 
   // ---- source
   std::ostringstream s;
-  const int HW = tile.TX + 2 * ks.H0, HH = tile.TY + 2 * ks.H1;
+  const int VX = tile.VX;
+  const int PAD = ks.H0 == 0 ? 0 : VX * ((ks.H0 + VX - 1) / VX);  // halo rounded up to whole vectors
+  const int SW = tile.TX + 2 * PAD, HH = tile.TY + 2 * ks.H1, WW = VX + 2 * PAD, WR = 2 * ks.H1 + 1;
   s << "#include \"mapx_dev.hiph\"\n";
   s << "namespace {\nstruct P {\n";
-  s << "  static constexpr int H0 = " << ks.H0 << ", H1 = " << ks.H1 << ", SW = " << HW << ";\n";
+  s << "  static constexpr int H0 = " << ks.H0 << ", H1 = " << ks.H1 << ", PAD = " << PAD << ", SW = " << SW << ";\n";
   s << "  struct Args {\n";
   for (const Param& p : ks.params) {
     if (p.kind == Param::InD2) s << "    mx::In<" << CT(p.dt) << "> i" << LN(p.node) << ";\n";
@@ -245,7 +248,12 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   }
   s << "  };\n";
   s << "  struct Smem {\n";
-  for (Node* st : staged) s << "    " << CT(st->meta.dt) << " sh" << LN(st) << "[" << HW * HH << "];\n";
+  for (Node* st : staged) s << "    alignas(16) " << CT(st->meta.dt) << " sh" << LN(st) << "[" << SW * HH << "];\n";
+  if (staged.empty()) s << "    int unused;\n";
+  s << "  };\n";
+  // per-thread register window of every staged node
+  s << "  struct Win {\n";
+  for (Node* st : staged) s << "    " << CT(st->meta.dt) << " w" << LN(st) << "[" << WR << "][" << WW << "];\n";
   if (staged.empty()) s << "    int unused;\n";
   s << "  };\n";
   // accumulators
@@ -275,8 +283,8 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
         o << "    const " << T << " " << V(n) << " = (" << V(n->prev[0]) << ") ? " << V(n->prev[1]) << " : "
           << V(n->prev[2]) << ";\n"; break;
       case Kind::Neighbor:
-        o << "    const " << T << " " << V(n) << " = sm->sh" << LN(n->prev[0]) << "[si + (" << n->coord.arr[1]
-          << ") * SW + (" << n->coord.arr[0] << ")];\n"; break;
+        o << "    const " << T << " " << V(n) << " = W.w" << LN(n->prev[0]) << "[H1 + (" << n->coord.arr[1]
+          << ")][PAD + v + (" << n->coord.arr[0] << ")];\n"; break;
       case Kind::Convolution: {
         // acc = 0; for y (outer), x (inner): acc += S[y][x] * W[y][x]  (FocalSkeleton.cpp:300-336)
         o << "    " << T << " " << V(n) << " = 0;\n";
@@ -284,7 +292,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
         for (int i1 = -h1; i1 <= h1; i1++)
           for (int i0 = -h0; i0 <= h0; i0++) {
             Variant w = quantize_literal(n->mask.at(i0 + h0, i1 + h1));
-            o << "    " << V(n) << " += sm->sh" << LN(n->prev[0]) << "[si + (" << i1 << ") * SW + (" << i0 << ")] * "
+            o << "    " << V(n) << " += W.w" << LN(n->prev[0]) << "[H1 + (" << i1 << ")][PAD + v + (" << i0 << ")] * "
               << lit(w) << ";\n";
           }
         break;
@@ -295,7 +303,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
         const int h0 = n->halo(0), h1 = n->halo(1);
         for (int i1 = -h1; i1 <= h1; i1++)
           for (int i0 = -h0; i0 <= h0; i0++) {
-            std::string sv = "sm->sh" + LN(n->prev[0]) + "[si + (" + std::to_string(i1) + ") * SW + (" +
+            std::string sv = "W.w" + LN(n->prev[0]) + "[H1 + (" + std::to_string(i1) + ")][PAD + v + (" +
                              std::to_string(i0) + ")]";
             if (n->op == MAX || n->op == MIN) {
               if (n->meta.dt == F32) o << "    " << V(n) << " = " << (n->op == MAX ? "fmaxf(" : "fminf(") << V(n) << ", " << sv << ");\n";
@@ -315,28 +323,42 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     (void)in_pre;
   };
 
-  // pre(): values of the staged nodes at one halo cell
-  s << "  __device__ static __forceinline__ void pre(const Args& a, const mx::Blk& k, int x, int y, bool inner, Smem& sm, int si) {\n";
+  // pre_vec(): values of the staged nodes at the VX halo cells x0v..x0v+VX-1 of row y
+  s << "  template <int VX> __device__ static __forceinline__ void pre_vec(const Args& a, const mx::Blk& k, int x0v, int y, Smem& sm, int si) {\n";
+  for (Node* n : all)
+    if (halo.count(n) && in_set.count(n) && n->meta.nd == D2)
+      s << "    " << CT(n->meta.dt) << " b" << LN(n) << "[VX];\n    mx::ldv<" << CT(n->meta.dt) << ", VX, H0>(a.i" << LN(n)
+        << ", k, x0v, y, b" << LN(n) << ");\n";
+  s << "#pragma unroll\n    for (int v = 0; v < VX; v++) {\n      const int x = x0v + v;\n";
   for (Node* n : all) {
     if (!halo.count(n)) continue;
     if (in_set.count(n)) {
       if (n->meta.nd == D0) s << "    const " << CT(n->meta.dt) << " " << V(n) << " = a.s" << LN(n) << ";\n";
-      else s << "    const " << CT(n->meta.dt) << " " << V(n) << " = inner ? mx::ldl(a.i" << LN(n) << ", k, x, y) : mx::ldf(a.i"
-             << LN(n) << ", k, x, y);\n";
+      else s << "    const " << CT(n->meta.dt) << " " << V(n) << " = b" << LN(n) << "[v];\n";
     } else emit_node(s, n, true);
   }
-  for (Node* st : staged) s << "    sm.sh" << LN(st) << "[si] = " << V(st) << ";\n";
-  s << "    (void)a; (void)k; (void)x; (void)y; (void)inner; (void)sm; (void)si;\n  }\n";
-  s << "  __device__ static __forceinline__ void pre_zero(Smem& sm, int si) {\n";
-  for (Node* st : staged) s << "    sm.sh" << LN(st) << "[si] = 0;\n";
+  for (Node* st : staged) s << "      sm.sh" << LN(st) << "[si + v] = " << V(st) << ";\n";
+  s << "      (void)x;\n    }\n    (void)a; (void)k; (void)y; (void)sm; (void)si;\n  }\n";
+  s << "  template <int VX> __device__ static __forceinline__ void pre_zero(Smem& sm, int si) {\n";
+  for (Node* st : staged) s << "#pragma unroll\n    for (int v = 0; v < VX; v++) sm.sh" << LN(st) << "[si + v] = 0;\n";
   s << "    (void)sm; (void)si;\n  }\n";
+  // load_win(): the thread's window rows from LDS with aligned vector reads
+  s << "  template <int VX> __device__ static __forceinline__ void load_win(const Smem& sm, int base, Win& W) {\n";
+  for (Node* st : staged) {
+    const std::string T = CT(st->meta.dt), L = LN(st);
+    s << "#pragma unroll\n    for (int r = 0; r < " << WR << "; r++)\n#pragma unroll\n      for (int c = 0; c < " << WW / VX
+      << "; c++) {\n        const mx::Vec<" << T << ", VX> t = *(const mx::Vec<" << T << ", VX>*)&sm.sh" << L
+      << "[base + r * SW + c * VX];\n#pragma unroll\n        for (int e = 0; e < VX; e++) W.w" << L
+      << "[r][c * VX + e] = t.v[e];\n      }\n";
+  }
+  s << "    (void)sm; (void)base; (void)W;\n  }\n";
 
   // post(): one output cell
-  s << "  __device__ static __forceinline__ void post(const Args& a, const mx::Blk& k, int x, int y, bool ok, const Smem* sm, int si, Acc& acc) {\n";
+  s << "  __device__ static __forceinline__ void post(const Args& a, const mx::Blk& k, int x, int y, bool ok, const Win& W, int v, Acc& acc) {\n";
   for (Node* n : all) {
     if (halo.count(n)) {
       if (!centre.count(n)) continue;
-      if (contains(staged, n)) s << "    const " << CT(n->meta.dt) << " " << V(n) << " = sm->sh" << LN(n) << "[si];\n";
+      if (contains(staged, n)) s << "    const " << CT(n->meta.dt) << " " << V(n) << " = W.w" << LN(n) << "[H1][PAD + v];\n";
       else if (in_set.count(n)) {
         if (n->meta.nd == D0) s << "    const " << CT(n->meta.dt) << " " << V(n) << " = a.s" << LN(n) << ";\n";
         else s << "    const " << CT(n->meta.dt) << " " << V(n) << " = mx::ldl(a.i" << LN(n) << ", k, x, y);\n";
@@ -362,7 +384,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
       s << "    mx::zone_step<" << CT(z.dt) << ", " << z.nzones << ">(acc.z" << j << ", ok, " << V(src) << ", (int)"
         << V(z.node->prev[1]) << ");\n";
   }
-  s << "    (void)a; (void)k; (void)x; (void)y; (void)ok; (void)sm; (void)si; (void)acc;\n  }\n";
+  s << "    (void)a; (void)k; (void)x; (void)y; (void)ok; (void)W; (void)v; (void)acc;\n  }\n";
   s << "};\n}  // namespace\n\n";
 
   // main kernel

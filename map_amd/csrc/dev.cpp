@@ -45,6 +45,14 @@ struct State {
   };
   std::vector<Rec> pending;
   std::unordered_map<std::string, hipGraphExec_t> graphs;  // by launch-sequence key
+  // per-launch timing
+  bool profile = false;
+  struct Ev {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Ev> evs;
+  std::vector<hipEvent_t> pool;
 };
 State& S() {
   static State s;
@@ -217,8 +225,53 @@ void launch(const Function& f, unsigned grid, unsigned block, void** params, con
     s.pending.push_back(std::move(r));
     return;
   }
+  State::Ev ev;
+  if (s.profile) {
+    for (hipEvent_t* e : {&ev.a, &ev.b}) {
+      if (!s.pool.empty()) { *e = s.pool.back(); s.pool.pop_back(); }
+      else check(hipEventCreate(e), "hipEventCreate");
+    }
+    ev.name = f.name;
+    check(hipEventRecord(ev.a, s.stream), "hipEventRecord");
+  }
   check(hipModuleLaunchKernel((hipFunction_t)f.fn, grid, 1, 1, block, 1, 1, 0, s.stream, params, nullptr),
         "hipModuleLaunchKernel");
+  if (s.profile) {
+    check(hipEventRecord(ev.b, s.stream), "hipEventRecord");
+    s.evs.push_back(ev);
+  }
+}
+
+void profile_set(bool on) { S().profile = on; }
+bool profile_on() { return S().profile; }
+
+std::vector<KernelTime> profile_take() {
+  State& s = S();
+  std::vector<KernelTime> out;
+  std::unordered_map<std::string, size_t> at;
+  for (auto& e : s.evs) {
+    check(hipEventSynchronize(e.b), "hipEventSynchronize");
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, e.a, e.b), "hipEventElapsedTime");
+    auto it = at.find(e.name);
+    if (it == at.end()) {
+      at[e.name] = out.size();
+      KernelTime k;
+      k.name = e.name;
+      k.min_ms = k.max_ms = ms;
+      out.push_back(k);
+      it = at.find(e.name);
+    }
+    KernelTime& k = out[it->second];
+    k.count++;
+    k.total_ms += ms;
+    if (ms < k.min_ms) k.min_ms = ms;
+    if (ms > k.max_ms) k.max_ms = ms;
+    s.pool.push_back(e.a);
+    s.pool.push_back(e.b);
+  }
+  s.evs.clear();
+  return out;
 }
 
 void record_begin() {

@@ -71,6 +71,17 @@ struct Timer {
   float ms();  // elapsed milliseconds (synchronises on the stop event)
 };
 
+// Per-launch kernel timing with HIP events around every direct launch on the
+// compute stream (launches replayed from graphs are not timed).
+struct KernelTime {
+  std::string name;
+  int count = 0;
+  double total_ms = 0, min_ms = 0, max_ms = 0;
+};
+void profile_set(bool on);
+bool profile_on();
+std::vector<KernelTime> profile_take();  // synchronises, returns and clears the timings
+
 // Number of compute units of the device (for grid sizing).
 int num_cus();
 

@@ -160,13 +160,17 @@ void Fuser::removeGroup(Group* g) {
 }
 
 bool Fuser::reaches(const Group* from, const Group* to) const {
-  std::vector<const Group*> stack(from->prev.begin(), from->prev.end());
-  std::set<const Group*> seen;
-  while (!stack.empty()) {
-    const Group* g = stack.back(); stack.pop_back();
+  // depth-first search over prev edges; groups are marked with a per-search stamp
+  const unsigned st = ++stamp_;
+  stack_.assign(from->prev.begin(), from->prev.end());
+  while (!stack_.empty()) {
+    const Group* g = stack_.back();
+    stack_.pop_back();
     if (g == to) return true;
-    if (!seen.insert(g).second) continue;
-    for (Group* p : g->prev) stack.push_back(p);
+    if (g->stamp == st) continue;
+    g->stamp = st;
+    for (Group* p : g->prev)
+      if (p->stamp != st) stack_.push_back(p);
   }
   return false;
 }

@@ -26,6 +26,7 @@ struct Group {
   int pat = NONE_PAT;
   std::vector<Group*> prev, next;
   std::vector<int> prev_pat, next_pat;
+  mutable unsigned stamp = 0;  // visit mark of reachability searches
 
   Meta meta() const;  // metadata of the highest-dimensional input or node (Group.cpp:143-156)
   int numdim() const { return meta().nd; }
@@ -86,6 +87,8 @@ class Fuser {
   std::unordered_map<Node*, std::vector<Node*>> next_;
   std::set<Group*> visited_;
   std::set<Node*> in_eval_;
+  mutable unsigned stamp_ = 0;
+  mutable std::vector<const Group*> stack_;
 };
 
 }  // namespace mapx

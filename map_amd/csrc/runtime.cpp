@@ -1,6 +1,7 @@
 #include "runtime.hpp"
 #include "dev.hpp"
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -13,6 +14,15 @@
 namespace mapx {
 
 static void fail(const std::string& m) { throw std::runtime_error(m); }
+
+namespace {
+struct HostTimer {
+  double& acc;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  explicit HostTimer(double& a) : acc(a) {}
+  ~HostTimer() { acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+};
+}  // namespace
 
 // ---- Store ---------------------------------------------------------------------
 
@@ -516,6 +526,8 @@ TileShape Runtime::tile_for(const Group& g) {
   // growing the height) or the height while it does
   const Meta m = g.meta();
   if (m.nd == D2) {
+    // vectors never straddle blocks: VX divides the block width
+    while (m.bs.arr[0] % t.VX) { t.VX /= 2; t.TX /= 2; }
     while (t.TX > m.bs.arr[0] && t.TX / t.VX > 16) { t.TX /= 2; t.TY *= 2; }
     while (t.TX > m.bs.arr[0] && t.VX > 1) { t.TX /= 2; t.VX /= 2; }
     while (t.TY > m.bs.arr[1] && t.TY > 256 / (t.TX / t.VX)) t.TY /= 2;
@@ -603,10 +615,14 @@ void Runtime::run_group(const Group& g, const std::vector<Node*>& targets,
       void* d = store.alloc(rasterBytes(n->meta));
       resident_[n] = d;
     }
-    TileShape tile = tile_for(g);
-    KernelSpec ks = generate_kernel(g, kouts, tile);
+    KernelEntry& ke = kernel_for(cur_plan_, g, kouts);
+    const KernelSpec& ks = ke.spec;
+    const TileShape tile = ks.tile;
     if (ks.form == Form::Focal) check_focal_geometry(gm, ks.H0, ks.H1);
-    dev::Function fn = dev::function(ks.source, ks.name);
+    dev::Function fn;
+    fn.fn = ke.fn;
+    fn.name = ks.name;
+    HostTimer ht_launch(cnt.ms_launch);
 
     const int BS0 = gm.bs.arr[0], BS1 = gm.bs.arr[1], NB0 = gm.nb(0), NB1 = gm.nb(1);
     const int TB0 = (BS0 + tile.TX - 1) / tile.TX, TB1 = (BS1 + tile.TY - 1) / tile.TY;
@@ -671,7 +687,9 @@ void Runtime::run_group(const Group& g, const std::vector<Node*>& targets,
       part0 += grid;
     }
     if (has_zonal) {
-      dev::Function fin = dev::function(ks.source, ks.fin_name);
+      dev::Function fin;
+      fin.fn = ke.fin;
+      fin.name = ks.fin_name;
       int np = (int)total_parts;
       void* fargs[3] = {&parts, &np, &res};
       size_t fsz[3] = {sizeof parts, sizeof np, sizeof res};
@@ -699,6 +717,30 @@ void Runtime::run_group(const Group& g, const std::vector<Node*>& targets,
     if (n->kind == Kind::Write) write_file(n);
 }
 
+std::string Runtime::plan_key(const std::vector<Node*>& list, const std::vector<Node*>& targets) const {
+  std::string k;
+  k.reserve(list.size() * 9 + targets.size() * 8 + 16);
+  auto put = [&](int v) { k.append((const char*)&v, sizeof v); };
+  put(opt.code_fusion); put(opt.focal_zonal); put(opt.tile.TX); put(opt.tile.TY); put(opt.tile.VX);
+  for (Node* n : list) { put(n->id); k.push_back(n->cut ? 'c' : (n->isOutput() ? 'o' : 'n')); }
+  k.push_back('|');
+  for (Node* t : targets) put(t->id);
+  return k;
+}
+
+Runtime::KernelEntry& Runtime::kernel_for(Plan* plan, const Group& g, const std::vector<Node*>& kouts) {
+  auto key = std::make_pair(&g, kouts);
+  auto it = plan->kernels.find(key);
+  if (it != plan->kernels.end()) return it->second;
+  HostTimer ht(cnt.ms_codegen);
+  KernelEntry e;
+  e.spec = generate_kernel(g, kouts, tile_for(g));
+  e.fn = dev::function(e.spec.source, e.spec.name).fn;
+  if (!e.spec.fin_name.empty()) e.fin = dev::function(e.spec.source, e.spec.fin_name).fn;
+  cnt.compiles++;
+  return plan->kernels.emplace(key, std::move(e)).first->second;
+}
+
 void Runtime::evaluate(const std::vector<Node*>& targets) {
   std::lock_guard<std::recursive_mutex> lk(mtx_);
   if (!dev::initialized()) fail("evaluate: devices not set up (call setupDevices first)");
@@ -718,7 +760,19 @@ void Runtime::evaluate(const std::vector<Node*>& targets) {
   FuseOptions fo;
   fo.code_fusion = opt.code_fusion;
   fo.focal_zonal = opt.focal_zonal;
-  auto groups = Fuser(fo).fuse(list);
+  const std::string pkey = plan_key(list, targets);
+  auto pit = plans_.find(pkey);
+  if (pit == plans_.end()) {
+    if (plans_.size() >= 64) plans_.clear();  // bounded: plans of old scripts are dropped
+    auto pl = std::make_unique<Plan>();
+    {
+      HostTimer ht(cnt.ms_fuse);
+      pl->groups = Fuser(fo).fuse(list);
+    }
+    pit = plans_.emplace(pkey, std::move(pl)).first;
+  }
+  cur_plan_ = pit->second.get();
+  const auto& groups = cur_plan_->groups;
 
   // consumers of each D2 raster in this evaluation, for early release
   std::map<Node*, int> uses;
@@ -759,7 +813,11 @@ std::string Runtime::plan(const std::vector<Node*>& targets) {
   FuseOptions fo;
   fo.code_fusion = opt.code_fusion;
   fo.focal_zonal = opt.focal_zonal;
-  auto groups = Fuser(fo).fuse(list);
+  std::vector<std::unique_ptr<Group>> groups;
+  {
+    HostTimer ht(cnt.ms_fuse);
+    groups = Fuser(fo).fuse(list);
+  }
   auto esc = [](const std::string& s) {
     std::string o;
     for (char c : s) {
@@ -795,7 +853,11 @@ std::string Runtime::plan(const std::vector<Node*>& targets) {
       bool has_zonal = false;
       for (Node* n : g->nodes) if (n->kind == Kind::ZonalReduc || n->kind == Kind::ZonalStats) has_zonal = true;
       if (!kouts.empty() || has_zonal) {
-        KernelSpec ks = generate_kernel(*g, kouts, tile_for(*g));
+        KernelSpec ks;
+        {
+          HostTimer ht(cnt.ms_codegen);
+          ks = generate_kernel(*g, kouts, tile_for(*g));
+        }
         j << ", \"kernel\": \"" << ks.name << "\", \"form\": " << (int)ks.form << ", \"halo\": [" << ks.H0 << ", " << ks.H1
           << "], \"source\": \"" << esc(ks.source) << "\"";
       }

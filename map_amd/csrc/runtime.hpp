@@ -36,6 +36,8 @@ struct Counters {
   long evals = 0, groups = 0, launches = 0, scalar_tasks = 0, compiles = 0;
   long bytes_h2d = 0, bytes_d2h = 0;
   double ms_last_eval = 0;
+  // host time per phase (the reference's Clock regions, runtime/Clock.hpp:24-27)
+  double ms_fuse = 0, ms_codegen = 0, ms_launch = 0, ms_eval_host = 0;
 };
 
 // Stream-ordered caching allocator: all device work runs on one in-order
@@ -88,6 +90,24 @@ class Runtime {
   std::vector<Node*> d2_outputs(const Group& g, const std::vector<Node*>& targets);
   TileShape tile_for(const Group& g);
   void write_file(Node* w);
+
+  // Plan cache: the fusion groups of an evaluation list and, per group and
+  // set of materialised outputs, its generated kernel. A repeated evaluation
+  // of the same sub-DAG (same nodes, same resident cut) reuses both (node ids
+  // are never reused, so a matching key means the same live nodes).
+  struct KernelEntry {
+    KernelSpec spec;
+    void* fn = nullptr;
+    void* fin = nullptr;
+  };
+  struct Plan {
+    std::vector<std::unique_ptr<Group>> groups;
+    std::map<std::pair<const Group*, std::vector<Node*>>, KernelEntry> kernels;
+  };
+  std::string plan_key(const std::vector<Node*>& list, const std::vector<Node*>& targets) const;
+  KernelEntry& kernel_for(Plan* plan, const Group& g, const std::vector<Node*>& kouts);
+  std::unordered_map<std::string, std::unique_ptr<Plan>> plans_;
+  Plan* cur_plan_ = nullptr;
 
   std::unordered_map<Node*, void*> resident_;
   std::recursive_mutex mtx_;

@@ -637,6 +637,14 @@ def stats():
 def last_eval_ms():
     return _lib.ma_lastEvalMs()
 
+
+def kernel_times():
+    """{kernel: {count, total_ms, min_ms, max_ms}} of launches timed since the last call ("profile" option)."""
+    s = _lib.ma_kernelTimes()
+    if s is None:
+        _err(None)
+    return _json.loads(s.decode())
+
 # ---- ctypes prototypes (python/map.py:723-830) ------------------------------------------
 
 
@@ -682,5 +690,6 @@ _proto('ma_plan', [ct.POINTER(Node), ct.c_int], ct.c_char_p)
 _proto('ma_compilePlan', [ct.POINTER(Node), ct.c_int], ct.c_int)
 _proto('ma_stats', [], ct.c_char_p)
 _proto('ma_lastEvalMs', [], ct.c_double)
+_proto('ma_kernelTimes', [], ct.c_char_p)
 _proto('ma_genDem', [Array, Array, ct.POINTER(ct.c_double), ct.c_int, ct.c_ulonglong], Node)
 _proto('ma_genZones', [Array, Array, ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), ct.c_int], Node)

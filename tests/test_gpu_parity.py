@@ -115,7 +115,7 @@ def test_local_integer_ops(G, O):
     def script(M, A, B):
         c = A * 3 + B
         d = (c > 100) * (A - 7) + (c <= 100) * B / 3
-        e = M.con(A < 128, A * A, A + 1)
+        e = M.con(A < 128, A * 2, A + 1)
         f = (B % 7) + (-A) + (A == 5)
         return d + e * 2 + f
     got = G.to_array(script(G, G.from_array(a, bs), G.from_array(b, bs)))

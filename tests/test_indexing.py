@@ -38,6 +38,9 @@ def ldf_index(ds, bs, nb, bx, by, x, y):
 
 def tile_for(bs, tx, ty, vx):
     """Mirror of Runtime::tile_for's clamping."""
+    while bs[0] % vx:
+        vx //= 2
+        tx //= 2
     while tx > bs[0] and tx // vx > 16:
         tx //= 2
         ty *= 2
@@ -49,23 +52,50 @@ def tile_for(bs, tx, ty, vx):
     return tx, ty, vx
 
 
+def ldv_reads(ds, bs, nb, bx, by, x, y, vx, h0):
+    """Global (block, x, y) reads of mx::ldv for the vector at (x..x+vx-1, y)."""
+    e0 = min(bs[0], ds[0] - bx * bs[0])
+    e1 = min(bs[1], ds[1] - by * bs[1])
+    xl = x + vx - 1
+    i0a = 0 if x < 0 else (2 if x >= e0 else 1)
+    i0b = 0 if xl < 0 else (2 if xl >= e0 else 1)
+    i1 = 0 if y < 0 else (2 if y >= e1 else 1)
+    if i0a == i0b:
+        nbx, nby = bx + i0a - 1, by + i1 - 1
+        if 0 <= nbx < nb[0] and 0 <= nby < nb[1]:
+            xx, yy = modulo(x, bs[0]), modulo(y, bs[1])
+            assert xx % vx == 0, "misaligned vector"
+            return [(nby * nb[0] + nbx, xx + v, yy, True) for v in range(vx)]
+    out = []
+    for v in range(vx):
+        xv = x + v
+        if -h0 <= xv < e0 + h0:
+            blk, sx, sy = ldf_index(ds, bs, nb, bx, by, xv, y)
+            out.append((blk, sx, sy, False))
+    return out
+
+
 GEOMS = [((512, 384), (128, 128), 1, 1, (256, 16, 4)),
          ((200, 150), (64, 32), 1, 1, (256, 16, 4)),
-         ((300, 220), (128, 64), 1, 1, (64, 16, 1)),
-         ((70, 41), (32, 16), 2, 2, (64, 16, 1)),
+         ((300, 220), (128, 64), 1, 1, (256, 16, 4)),
+         ((70, 41), (32, 16), 2, 2, (256, 16, 4)),
          ((33, 17), (16, 16), 1, 1, (256, 16, 4)),
-         ((40, 40), (40, 40), 3, 3, (64, 16, 1))]
+         ((40, 40), (40, 40), 3, 3, (256, 16, 4)),
+         ((101, 57), (50, 30), 1, 1, (256, 16, 4)),
+         ((98, 60), (49, 20), 2, 1, (256, 16, 4))]
 
 
 @pytest.mark.parametrize("ds,bs,h0,h1,tile", GEOMS)
 def test_focal_reads_in_bounds(ds, bs, h0, h1, tile):
     tx, ty, vx = tile_for(bs, *tile)
     twx = tx // vx
-    assert 256 % twx == 0 and ty % (256 // twx) == 0
+    assert 256 % twx == 0 and ty % (256 // twx) == 0 and bs[0] % vx == 0
+    pad = 0 if h0 == 0 else vx * ((h0 + vx - 1) // vx)
+    sw, hh = tx + 2 * pad, ty + 2 * h1
+    nv = sw // vx
     nb = ((ds[0] + bs[0] - 1) // bs[0], (ds[1] + bs[1] - 1) // bs[1])
     total = nb[0] * nb[1] * bs[0] * bs[1]
     tb = ((bs[0] + tx - 1) // tx, (bs[1] + ty - 1) // ty)
-    hw, hh = tx + 2 * h0, ty + 2 * h1
     for b in range(nb[0] * nb[1]):
         bx, by = b % nb[0], b // nb[0]
         e0 = min(bs[0], ds[0] - bx * bs[0])
@@ -75,17 +105,24 @@ def test_focal_reads_in_bounds(ds, bs, h0, h1, tile):
             x0, y0 = (t % tb[0]) * tx, (t // tb[0]) * ty
             if x0 >= e0 or y0 >= e1:
                 continue
-            for ex, ey in itertools.product(range(hw), range(hh)):
-                x, y = x0 - h0 + ex, y0 - h1 + ey
-                if not (x < e0 + h0 and y < e1 + h1):
+            for r, j in itertools.product(range(hh), range(nv)):
+                x, y = x0 - pad + j * vx, y0 - h1 + r
+                if not (y < e1 + h1 and x + vx > -h0 and x < e0 + h0):
                     continue
-                blk, sx, sy = ldf_index(ds, bs, nb, bx, by, x, y)
-                assert 0 <= sx < bs[0] and 0 <= sy < bs[1], (b, t, x, y, sx, sy)
-                be0 = min(bs[0], ds[0] - (blk % nb[0]) * bs[0])
-                be1 = min(bs[1], ds[1] - (blk // nb[0]) * bs[1])
-                assert sx < be0 and sy < be1, "read of padding at %s" % ((b, x, y),)
-                idx = (blk * bs[1] + sy) * bs[0] + sx
-                assert 0 <= idx < total
+                for blk, sx, sy, vec in ldv_reads(ds, bs, nb, bx, by, x, y, vx, h0):
+                    assert 0 <= sx < bs[0] and 0 <= sy < bs[1], (b, t, x, y, sx, sy)
+                    idx = (blk * bs[1] + sy) * bs[0] + sx
+                    assert 0 <= idx < total
+                    if not vec:  # border-rule reads land on valid data
+                        be0 = min(bs[0], ds[0] - (blk % nb[0]) * bs[0])
+                        be1 = min(bs[1], ds[1] - (blk // nb[0]) * bs[1])
+                        assert sx < be0 and sy < be1, "read of padding at %s" % ((b, x, y),)
+            # LDS window reads of every thread stay inside the halo tile
+            for tid in range(256):
+                ly, lx = tid // twx, (tid % twx) * vx
+                for ry in range(ty // (256 // twx)):
+                    base = (ly + ry * (256 // twx)) * sw + lx
+                    assert base + 2 * h1 * sw + vx + 2 * pad - 1 < hh * sw
             # per-cell loads use clamped coordinates
             for lx, ly in itertools.product(range(tx), range(ty)):
                 x, y = x0 + lx, y0 + ly

@@ -61,7 +61,7 @@ def test_mask_literals_match_reference(M, ref):
         r = M.from_array(np.zeros((8, 8), np.float32), (8, 8))
         c = M.convolve(r, [[float(np.float32(float(f))), 0.0, 0.0], [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
         src = M.plan(c)["groups"][-1]["source"]
-        lit = re.search(r"\+= sm->sh\d+\[si \+ \(-1\) \* SW \+ \(-1\)\] \* ([^;]+);", src).group(1)
+        lit = re.search(r"\+= W\.w\d+\[H1 \+ \(-1\)\]\[PAD \+ v \+ \(-1\)\] \* ([^;]+);", src).group(1)
         assert np.float32(float.fromhex(lit.rstrip("f"))) == want, (f, lit)
 
 
