@@ -74,6 +74,9 @@ def main():
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-gens", type=int, default=8, help="generations timed once in the plain form")
+    ap.add_argument("--tune", type=int, default=1, help="autotune the tile shape before timing")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a U8 copy of the input raster (a local cast kernel), for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -86,6 +89,24 @@ def main():
     S = M.from_array(s0, bs)
     M.set_option("form", 1 if args.form == "batched" else 0)
     M.set_option("graph", args.graph)
+
+    if args.copy_calib:
+        cp = M.astype(S, M.U8)  # one read and one write of every cell
+        for _ in range(max(1, args.steps)):
+            M.release(cp)
+            M.eval(cp)
+        print(json.dumps({"copy_calib_bytes_read": args.side * args.side, "copy_calib_bytes_written": args.side * args.side,
+                          "launches": max(1, args.steps)}), flush=True)
+        return
+
+    tuned = None
+    if args.tune:
+        # the tile shape is tuned on a 10-generation script over the same raster:
+        # every generation is the same fused group, so the choice carries over
+        tune_out = life_script(M, S, 10)
+        rep = M.autotune(tune_out, reps=2)
+        tuned = rep["chosen"]
+        del tune_out
 
     out = life_script(M, S, args.gens)
 
@@ -144,6 +165,7 @@ def main():
                      "kernel": kname, "kernel_avg_ms": round(kern_ms, 5), "kernel_min_ms": round(krec["min_ms"], 5),
                      "launches": launches, "algorithmic_bytes_per_launch": int(alg_bytes)},
         "device_ms_per_step": round(min(dev_ms), 3),
+        "tuned_tile": tuned,
         "host_ms": {"fuse": round(host["ms_fuse"], 1), "codegen_compile": round(host["ms_codegen"], 1),
                     "launch": round(host["ms_launch"], 1)},
     }

@@ -144,6 +144,13 @@ double ma_lastEvalMs(void);
 /* Durations of the kernels launched since the last call while the "profile"
  * option is set (HIP events around each launch), as JSON by kernel name. */
 const char *ma_kernelTimes(void);
+/* Autotuner: evaluates the nodes once per candidate tile shape (width,
+ * height, cells per thread: also the layout of the LDS halo tile), times each
+ * launch with HIP events, and keeps the fastest shape per fused-group
+ * signature and block size for later evaluations. Returns a JSON report. */
+const char *ma_autotune(ma_node_t **vec, int num, int reps);
+/* Loads (save = 0) or writes (save = 1) the tuning results; returns their count. */
+int ma_tuneFile(const char *path, int save);
 /* Fusion legality of a pattern pair (PatternEnum bit sets): bit 0 pipe, bit 1
  * flat (runtime/Pattern.cpp:53-182 as applied by the fuser). */
 int ma_fusionRules(int top_or_left, int bot_or_right);

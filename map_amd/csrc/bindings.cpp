@@ -227,7 +227,7 @@ int ma_setOption(const char* key, long long value) {
     else if (k == "tile_ty") o.tile.TY = (int)value;
     else if (k == "tile_vx") o.tile.VX = (int)value;
     else if (k == "wg_per_cu") {
-      if (value < 1 || value > 64) throw std::runtime_error("setOption: wg_per_cu in [1, 64]");
+      if (value < 0 || value > 64) throw std::runtime_error("setOption: wg_per_cu in [0, 64]");
       o.wg_per_cu = (int)value;
     } else throw std::runtime_error("setOption: unknown option " + k);
     return 0;
@@ -313,6 +313,25 @@ const char* ma_kernelTimes(void) {
     g_json = s.str();
     return g_json.c_str();
   }, (const char*)nullptr);
+}
+
+const char* ma_autotune(ma_node_t** vec, int num, int reps) {
+  return guard([&] {
+    if (num <= 0 || !vec) throw std::runtime_error("autotune: no nodes");
+    std::vector<Node*> v;
+    for (int i = 0; i < num; i++) v.push_back(need(vec[i]));
+    g_json = R().autotune(v, reps);
+    return g_json.c_str();
+  }, (const char*)nullptr);
+}
+
+int ma_tuneFile(const char* path, int save) {
+  return guard([&] {
+    if (!path) throw std::runtime_error("tuneFile: null path");
+    if (save) R().tune_save(path);
+    else R().tune_load(path);
+    return (int)R().tuned_count();
+  }, -1);
 }
 
 int ma_fusionRules(int top_or_left, int bot_or_right) {

@@ -256,6 +256,15 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   for (Node* st : staged) s << "    " << CT(st->meta.dt) << " w" << LN(st) << "[" << WR << "][" << WW << "];\n";
   if (staged.empty()) s << "    int unused;\n";
   s << "  };\n";
+  // the thread's VX cells of every input read per cell
+  s << "  struct Cells {\n";
+  for (Node* n : all)
+    if (cell_inputs.count(n) && n->meta.nd == D2) s << "    " << CT(n->meta.dt) << " c" << LN(n) << "[" << VX << "];\n";
+  s << "    int unused;\n  };\n";
+  // one cell's outputs
+  s << "  struct Out {\n";
+  for (Node* o : d2_outs) s << "    " << CT(o->meta.dt) << " o" << LN(o) << ";\n";
+  s << "    int unused;\n  };\n";
   // accumulators
   s << "  struct Acc {\n";
   for (size_t j = 0; j < ks.zouts.size(); j++) {
@@ -323,25 +332,53 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     (void)in_pre;
   };
 
-  // pre_vec(): values of the staged nodes at the VX halo cells x0v..x0v+VX-1 of row y
-  s << "  template <int VX> __device__ static __forceinline__ void pre_vec(const Args& a, const mx::Blk& k, int x0v, int y, Smem& sm, int si) {\n";
-  for (Node* n : all)
-    if (halo.count(n) && in_set.count(n) && n->meta.nd == D2)
-      s << "    " << CT(n->meta.dt) << " b" << LN(n) << "[VX];\n    mx::ldv<" << CT(n->meta.dt) << ", VX, H0>(a.i" << LN(n)
-        << ", k, x0v, y, b" << LN(n) << ");\n";
-  s << "#pragma unroll\n    for (int v = 0; v < VX; v++) {\n      const int x = x0v + v;\n";
-  for (Node* n : all) {
-    if (!halo.count(n)) continue;
-    if (in_set.count(n)) {
-      if (n->meta.nd == D0) s << "    const " << CT(n->meta.dt) << " " << V(n) << " = a.s" << LN(n) << ";\n";
-      else s << "    const " << CT(n->meta.dt) << " " << V(n) << " = b" << LN(n) << "[v];\n";
-    } else emit_node(s, n, true);
-  }
-  for (Node* st : staged) s << "      sm.sh" << LN(st) << "[si + v] = " << V(st) << ";\n";
-  s << "      (void)x;\n    }\n    (void)a; (void)k; (void)y; (void)sm; (void)si;\n  }\n";
+  // pre_vec() / pre_row(): values of the staged nodes at the VX halo cells
+  // x0v..x0v+VX-1 of row y (inputs read with ldv's border rule, or through a
+  // known row source), stored to LDS as vectors
+  auto pre_body = [&](const char* fname, bool row) {
+    s << "  template <int VX> __device__ static __forceinline__ void " << fname << "(const Args& a, const mx::Blk& k, "
+      << (row ? "const mx::RowSrc& rs, " : "") << "int x0v, int y, Smem& sm, int si) {\n";
+    for (Node* n : all)
+      if (halo.count(n) && in_set.count(n) && n->meta.nd == D2) {
+        s << "    " << CT(n->meta.dt) << " b" << LN(n) << "[VX];\n";
+        if (row) s << "    mx::ldrow<" << CT(n->meta.dt) << ", VX>(a.i" << LN(n) << ", rs, x0v, b" << LN(n) << ");\n";
+        else s << "    mx::ldv<" << CT(n->meta.dt) << ", VX, H0>(a.i" << LN(n) << ", k, x0v, y, b" << LN(n) << ");\n";
+      }
+    for (Node* st : staged) s << "    " << CT(st->meta.dt) << " t" << LN(st) << "[VX];\n";
+    s << "#pragma unroll\n    for (int v = 0; v < VX; v++) {\n      const int x = x0v + v;\n";
+    for (Node* n : all) {
+      if (!halo.count(n)) continue;
+      if (in_set.count(n)) {
+        if (n->meta.nd == D0) s << "    const " << CT(n->meta.dt) << " " << V(n) << " = a.s" << LN(n) << ";\n";
+        else s << "    const " << CT(n->meta.dt) << " " << V(n) << " = b" << LN(n) << "[v];\n";
+      } else emit_node(s, n, true);
+    }
+    for (Node* st : staged) s << "      t" << LN(st) << "[v] = " << V(st) << ";\n";
+    s << "      (void)x;\n    }\n";
+    for (Node* st : staged) s << "    mx::stvec<" << CT(st->meta.dt) << ", VX>(&sm.sh" << LN(st) << "[si], t" << LN(st) << ");\n";
+    s << "    (void)a; (void)k; (void)y; (void)sm; (void)si;\n  }\n";
+  };
+  pre_body("pre_vec", false);
+  pre_body("pre_row", true);
   s << "  template <int VX> __device__ static __forceinline__ void pre_zero(Smem& sm, int si) {\n";
-  for (Node* st : staged) s << "#pragma unroll\n    for (int v = 0; v < VX; v++) sm.sh" << LN(st) << "[si + v] = 0;\n";
+  for (Node* st : staged)
+    s << "    { " << CT(st->meta.dt) << " z[VX] = {}; mx::stvec<" << CT(st->meta.dt) << ", VX>(&sm.sh" << LN(st) << "[si], z); }\n";
   s << "    (void)sm; (void)si;\n  }\n";
+  // load_cells(): the thread's VX cells of every input read per cell
+  s << "  template <int VX> __device__ static __forceinline__ void load_cells(const Args& a, mx::i64 off, Cells& C) {\n";
+  for (Node* n : all)
+    if (cell_inputs.count(n) && n->meta.nd == D2)
+      s << "    mx::ldcell<" << CT(n->meta.dt) << ", VX>(a.i" << LN(n) << ", off, C.c" << LN(n) << ");\n";
+  s << "    (void)a; (void)off; (void)C;\n  }\n";
+  // store(): the thread's VX output cells as vectors (nv < VX: the first nv cells)
+  s << "  template <int VX> __device__ static __forceinline__ void store(const Args& a, mx::i64 off, int nv, const Out (&o)[VX]) {\n";
+  for (Node* on : d2_outs) {
+    const std::string T = CT(on->meta.dt), L = LN(on);
+    s << "    {\n      " << T << " t[VX];\n#pragma unroll\n      for (int v = 0; v < VX; v++) t[v] = o[v].o" << L << ";\n"
+      << "      if (nv == VX) mx::stvec<" << T << ", VX>(a.o" << L << " + off, t);\n"
+      << "      else for (int v = 0; v < nv; v++) a.o" << L << "[off + v] = t[v];\n    }\n";
+  }
+  s << "    (void)a; (void)off; (void)nv; (void)o;\n  }\n";
   // load_win(): the thread's window rows from LDS with aligned vector reads
   s << "  template <int VX> __device__ static __forceinline__ void load_win(const Smem& sm, int base, Win& W) {\n";
   for (Node* st : staged) {
@@ -354,7 +391,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   s << "    (void)sm; (void)base; (void)W;\n  }\n";
 
   // post(): one output cell
-  s << "  __device__ static __forceinline__ void post(const Args& a, const mx::Blk& k, int x, int y, bool ok, const Win& W, int v, Acc& acc) {\n";
+  s << "  __device__ static __forceinline__ void post(const Args& a, const mx::Blk& k, int x, int y, bool ok, const Win& W, const Cells& C, int v, Acc& acc, Out& out) {\n";
   for (Node* n : all) {
     if (halo.count(n)) {
       if (!centre.count(n)) continue;
@@ -368,13 +405,12 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     if (in_set.count(n)) {
       if (!cell_inputs.count(n)) continue;
       if (n->meta.nd == D0) s << "    const " << CT(n->meta.dt) << " " << V(n) << " = a.s" << LN(n) << ";\n";
-      else s << "    const " << CT(n->meta.dt) << " " << V(n) << " = mx::ldl(a.i" << LN(n) << ", k, x, y);\n";
+      else s << "    const " << CT(n->meta.dt) << " " << V(n) << " = C.c" << LN(n) << "[v];\n";
       continue;
     }
     emit_node(s, n, false);
   }
-  for (Node* o : d2_outs)
-    s << "    if (ok) a.o" << LN(o) << "[k.off + (mx::i64)y * k.BS0 + x] = " << V(o) << ";\n";
+  for (Node* o : d2_outs) s << "    out.o" << LN(o) << " = " << V(o) << ";\n";
   for (size_t j = 0; j < ks.zouts.size(); j++) {
     const ZonalOut& z = ks.zouts[j];
     Node* src = z.node->prev[0];
@@ -384,7 +420,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
       s << "    mx::zone_step<" << CT(z.dt) << ", " << z.nzones << ">(acc.z" << j << ", ok, " << V(src) << ", (int)"
         << V(z.node->prev[1]) << ");\n";
   }
-  s << "    (void)a; (void)k; (void)x; (void)y; (void)ok; (void)W; (void)v; (void)acc;\n  }\n";
+  s << "    (void)a; (void)k; (void)x; (void)y; (void)ok; (void)W; (void)C; (void)v; (void)acc; (void)out;\n  }\n";
   s << "};\n}  // namespace\n\n";
 
   // main kernel

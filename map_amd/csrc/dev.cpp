@@ -123,6 +123,13 @@ void* stream() {
 }
 int num_cus() { return S().cus; }
 
+int occupancy(const Function& f, int block) {
+  int n = 0;
+  check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&n, (hipFunction_t)f.fn, block, 0),
+        "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+  return n > 0 ? n : 1;
+}
+
 void* malloc_device(size_t bytes) {
   void* p = nullptr;
   check(hipMalloc(&p, bytes ? bytes : 1), "hipMalloc");

@@ -84,6 +84,8 @@ std::vector<KernelTime> profile_take();  // synchronises, returns and clears the
 
 // Number of compute units of the device (for grid sizing).
 int num_cus();
+// Work-groups of `block` threads of this kernel that fit on one CU at once.
+int occupancy(const Function& f, int block);
 
 const std::string& header_source();  // text of mapx_dev.hiph embedded at build time
 std::string cache_dir();

@@ -396,6 +396,7 @@ void Runtime::setupDevices(const std::string& plat, unsigned dev_type, const std
   if (const char* e = std::getenv("MAPX_DEVICE")) device = std::atoi(e);
   dev::init(device);
   read_env(opt);
+  if (const char* p = std::getenv("MAPX_TUNE_CACHE")) tune_load(p);
 }
 
 size_t Runtime::rasterBytes(const Meta& m) const {
@@ -520,8 +521,18 @@ std::vector<Node*> Runtime::d2_outputs(const Group& g, const std::vector<Node*>&
   return v;
 }
 
+std::string Runtime::tune_key(const Group& g) const {
+  const Meta m = g.meta();
+  return g.signature() + "#" + std::to_string(m.bs.arr[0]) + "x" + std::to_string(m.bs.arr[1]);
+}
+
 TileShape Runtime::tile_for(const Group& g) {
-  TileShape t = opt.tile.TX > 0 ? opt.tile : default_tile(g);
+  TileShape t = default_tile(g);
+  if (opt.tile.TX > 0) t = opt.tile;
+  else {
+    auto it = tuned_.find(tune_key(g));
+    if (it != tuned_.end()) t = it->second;
+  }
   // a tile never exceeds its block: halve the width (keeping 256 threads by
   // growing the height) or the height while it does
   const Meta m = g.meta();
@@ -626,7 +637,10 @@ void Runtime::run_group(const Group& g, const std::vector<Node*>& targets,
 
     const int BS0 = gm.bs.arr[0], BS1 = gm.bs.arr[1], NB0 = gm.nb(0), NB1 = gm.nb(1);
     const int TB0 = (BS0 + tile.TX - 1) / tile.TX, TB1 = (BS1 + tile.TY - 1) / tile.TY;
-    const int gmax = std::max(8, (dev::num_cus() * opt.wg_per_cu) & ~7);
+    // persistent grid: every work-group resident at once (no tail wave), a
+    // multiple of 8 for the XCD-contiguous work split
+    const int per_cu = opt.wg_per_cu > 0 ? std::min(opt.wg_per_cu, ke.occ) : ke.occ;
+    const int gmax = std::max(8, (dev::num_cus() * per_cu) & ~7);
     auto grid_for = [&](long work) {
       long gsz = std::min<long>(work, gmax);
       return (int)((gsz + 7) & ~7L);
@@ -735,7 +749,10 @@ Runtime::KernelEntry& Runtime::kernel_for(Plan* plan, const Group& g, const std:
   HostTimer ht(cnt.ms_codegen);
   KernelEntry e;
   e.spec = generate_kernel(g, kouts, tile_for(g));
-  e.fn = dev::function(e.spec.source, e.spec.name).fn;
+  kernel_group_[e.spec.name] = tune_key(g);
+  dev::Function f = dev::function(e.spec.source, e.spec.name);
+  e.fn = f.fn;
+  e.occ = dev::occupancy(f, 256);
   if (!e.spec.fin_name.empty()) e.fin = dev::function(e.spec.source, e.spec.fin_name).fn;
   cnt.compiles++;
   return plan->kernels.emplace(key, std::move(e)).first->second;
@@ -866,6 +883,87 @@ std::string Runtime::plan(const std::vector<Node*>& targets) {
   }
   j << "]}";
   return j.str();
+}
+
+std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
+  std::lock_guard<std::recursive_mutex> lk(mtx_);
+  if (reps < 1) reps = 1;
+  static const TileShape cands[] = {{256, 16, 4}, {256, 32, 4}, {256, 8, 4}, {128, 32, 4}, {128, 16, 4},
+                                    {512, 16, 4}, {128, 16, 2}, {256, 16, 2}, {64, 16, 1}, {64, 32, 1}};
+  const TileShape saved = opt.tile;
+  const bool saved_graph = opt.graph, saved_prof = dev::profile_on();
+  opt.graph = false;
+  // best average kernel time per tune key
+  std::map<std::string, std::pair<double, TileShape>> best;
+  std::ostringstream rep;
+  rep << "{\"candidates\": [";
+  bool first = true;
+  try {
+    for (const TileShape& c : cands) {
+      opt.tile = c;
+      for (Node* t : targets) drop(t);
+      evaluate(targets);  // compile and warm
+      dev::profile_set(true);
+      dev::profile_take();
+      for (int r = 0; r < reps; r++) {
+        for (Node* t : targets) drop(t);
+        evaluate(targets);
+      }
+      auto times = dev::profile_take();
+      dev::profile_set(false);
+      for (auto& k : times) {
+        auto kg = kernel_group_.find(k.name);
+        if (kg == kernel_group_.end()) continue;
+        const double avg = k.total_ms / k.count;
+        auto b = best.find(kg->second);
+        if (b == best.end() || avg < b->second.first) best[kg->second] = {avg, c};
+        rep << (first ? "" : ", ") << "{\"tile\": [" << c.TX << ", " << c.TY << ", " << c.VX << "], \"kernel\": \""
+            << k.name << "\", \"avg_ms\": " << avg << "}";
+        first = false;
+      }
+    }
+  } catch (...) {
+    opt.tile = saved;
+    opt.graph = saved_graph;
+    dev::profile_set(saved_prof);
+    throw;
+  }
+  opt.tile = saved;
+  opt.graph = saved_graph;
+  dev::profile_set(saved_prof);
+  rep << "], \"chosen\": [";
+  first = true;
+  for (auto& b : best) {
+    tuned_[b.first] = b.second.second;
+    rep << (first ? "" : ", ") << "{\"avg_ms\": " << b.second.first << ", \"tile\": [" << b.second.second.TX << ", "
+        << b.second.second.TY << ", " << b.second.second.VX << "]}";
+    first = false;
+  }
+  rep << "]}";
+  for (Node* t : targets) drop(t);
+  if (const char* p = std::getenv("MAPX_TUNE_CACHE")) tune_save(p);
+  return rep.str();
+}
+
+// The tuning cache file: one line per entry, "TX TY VX <tune key>".
+void Runtime::tune_load(const std::string& path) {
+  std::ifstream f(path);
+  if (!f) return;
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream l(line);
+    TileShape t;
+    std::string key;
+    if (!(l >> t.TX >> t.TY >> t.VX)) continue;
+    std::getline(l, key);
+    if (!key.empty() && key[0] == ' ') key.erase(0, 1);
+    if (!key.empty() && t.TX > 0 && t.VX > 0 && t.TY > 0) tuned_[key] = t;
+  }
+}
+
+void Runtime::tune_save(const std::string& path) const {
+  std::ofstream f(path, std::ios::trunc);
+  for (auto& e : tuned_) f << e.second.TX << " " << e.second.TY << " " << e.second.VX << " " << e.first << "\n";
 }
 
 }  // namespace mapx

@@ -28,7 +28,7 @@ struct Options {
   bool graph = false;         // replay identical launch sequences from hipGraphs
   bool focal_zonal = false;   // fuse FOCAL -> ZONAL into one kernel
   bool code_fusion = true;
-  int wg_per_cu = 8;          // grid size cap (work-groups per CU)
+  int wg_per_cu = 0;          // grid size cap in work-groups per CU (0: the kernel's occupancy)
   TileShape tile{0, 0, 0};    // forced tile shape (0 = per-group default / tuned)
 };
 
@@ -81,6 +81,15 @@ class Runtime {
   // Plan of an evaluation without executing it (groups and kernel sources), as JSON.
   std::string plan(const std::vector<Node*>& targets);
 
+  // Autotuner: evaluates `targets` once per candidate tile shape (TX, TY, VX),
+  // times every launch with HIP events and keeps, per fused-group signature
+  // and block size, the fastest shape; tile selection consults the result.
+  // Returns a JSON report of every candidate's average kernel time.
+  std::string autotune(const std::vector<Node*>& targets, int reps);
+  void tune_load(const std::string& path);
+  void tune_save(const std::string& path) const;
+  size_t tuned_count() const { return tuned_.size(); }
+
  private:
   struct GroupRun;
   std::vector<Node*> cut_list(const std::vector<Node*>& targets);
@@ -99,6 +108,7 @@ class Runtime {
     KernelSpec spec;
     void* fn = nullptr;
     void* fin = nullptr;
+    int occ = 0;  // resident work-groups per CU
   };
   struct Plan {
     std::vector<std::unique_ptr<Group>> groups;
@@ -108,6 +118,9 @@ class Runtime {
   KernelEntry& kernel_for(Plan* plan, const Group& g, const std::vector<Node*>& kouts);
   std::unordered_map<std::string, std::unique_ptr<Plan>> plans_;
   Plan* cur_plan_ = nullptr;
+  std::string tune_key(const Group& g) const;
+  std::map<std::string, TileShape> tuned_;
+  std::map<std::string, std::string> kernel_group_;  // kernel name -> tune key
 
   std::unordered_map<Node*, void*> resident_;
   std::recursive_mutex mtx_;

@@ -630,6 +630,22 @@ def compile_plan(*args):
     return r
 
 
+def autotune(*args, reps=3):
+    """Times candidate tile shapes for the groups of an evaluation and keeps the fastest."""
+    ptr, n, keep = _nodes(args)
+    s = _lib.ma_autotune(ptr, n, int(reps))
+    if not s:
+        _err(None)
+    return _json.loads(s.decode())
+
+
+def tune_file(path, save=False):
+    r = _lib.ma_tuneFile(path.encode(), 1 if save else 0)
+    if r < 0:
+        _err(None)
+    return r
+
+
 def stats():
     return _json.loads(_lib.ma_stats().decode())
 
@@ -691,5 +707,7 @@ _proto('ma_compilePlan', [ct.POINTER(Node), ct.c_int], ct.c_int)
 _proto('ma_stats', [], ct.c_char_p)
 _proto('ma_lastEvalMs', [], ct.c_double)
 _proto('ma_kernelTimes', [], ct.c_char_p)
+_proto('ma_autotune', [ct.POINTER(Node), ct.c_int, ct.c_int], ct.c_char_p)
+_proto('ma_tuneFile', [ct.c_char_p, ct.c_int], ct.c_int)
 _proto('ma_genDem', [Array, Array, ct.POINTER(ct.c_double), ct.c_int, ct.c_ulonglong], Node)
 _proto('ma_genZones', [Array, Array, ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), ct.c_int], Node)

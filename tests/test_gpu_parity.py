@@ -77,7 +77,8 @@ def test_hillshade(G, O, w, h, bs):
     assert np.abs(got.astype(np.float64) - ref64).max() <= 2 * np.abs(ref32.astype(np.float64) - ref64).max() + 1e-6
 
 
-@pytest.mark.parametrize("w,h,bs,gens", [(200, 150, (64, 32), 12), (512, 512, (128, 128), 40), (1024, 768, (256, 256), 25)])
+@pytest.mark.parametrize("w,h,bs,gens", [(200, 150, (64, 32), 12), (130, 90, (40, 40), 9), (512, 512, (128, 128), 40),
+                                         (1024, 768, (256, 256), 25)])
 def test_life(G, O, w, h, bs, gens):
     print("case life", w, h, bs, gens, flush=True)
     rng = np.random.Generator(np.random.PCG64(1))

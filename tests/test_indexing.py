@@ -1,8 +1,9 @@
 """Host-side check of the focal template's global indexing.
 
-Emulates, for small raster geometries, every global read the focal kernel
-body makes (device/mapx_dev.hiph: tiles of each block, halo cells inside
-[-H, E+H), the block-relative border rule of ldf, clamped per-cell loads)
+Emulates, for small raster geometries, every global access the focal kernel
+body makes (device/mapx_dev.hiph: tiles of each block, body vectors through
+row_src, pad vectors through ldv's border rule inside [-H, E+H), per-thread
+cell vectors and output stores)
 and checks that each lands inside the raster allocation and inside the block
 it claims. Geometries include tiles wider than blocks and partial edge
 blocks (the cases that would read out of bounds without the halo bound).
@@ -105,15 +106,38 @@ def test_focal_reads_in_bounds(ds, bs, h0, h1, tile):
             x0, y0 = (t % tb[0]) * tx, (t // tb[0]) * ty
             if x0 >= e0 or y0 >= e1:
                 continue
-            for r, j in itertools.product(range(hh), range(nv)):
-                x, y = x0 - pad + j * vx, y0 - h1 + r
-                if not (y < e1 + h1 and x + vx > -h0 and x < e0 + h0):
-                    continue
-                for blk, sx, sy, vec in ldv_reads(ds, bs, nb, bx, by, x, y, vx, h0):
+            body_fast = x0 + tx <= e0
+            pv = pad // vx
+            vecs = [(r, x0 + j * vx, True) for r in range(hh) for j in range(tx // vx)]
+            vecs += [(r, x0 - pad + j * vx if j < pv else x0 + tx + (j - pv) * vx, False)
+                     for r in range(hh) for j in range(2 * pv)]
+            for r, x, body in vecs:
+                y = y0 - h1 + r
+                if body:
+                    if y >= e1 + h1 or x >= e0 + h0:
+                        continue
+                    if body_fast:  # row_src: the row's block (central, above/below or mirrored row)
+                        assert 0 <= x and x + vx <= e0
+                        if 0 <= y < e1:
+                            blk, yy = b, y
+                        elif 0 <= by + (-1 if y < 0 else 1) < nb[1]:
+                            blk, yy = (by + (-1 if y < 0 else 1)) * nb[0] + bx, modulo(y, bs[1])
+                        else:
+                            blk, yy = b, invert(y, e1)
+                        be1 = min(bs[1], ds[1] - (blk // nb[0]) * bs[1])
+                        assert 0 <= yy < be1
+                        reads = [(blk, x + v, yy, False) for v in range(vx)]
+                    else:
+                        reads = ldv_reads(ds, bs, nb, bx, by, x, y, vx, h0)
+                else:
+                    if not (y < e1 + h1 and x + vx > -h0 and x < e0 + h0):
+                        continue
+                    reads = ldv_reads(ds, bs, nb, bx, by, x, y, vx, h0)
+                for blk, sx, sy, vec in reads:
                     assert 0 <= sx < bs[0] and 0 <= sy < bs[1], (b, t, x, y, sx, sy)
                     idx = (blk * bs[1] + sy) * bs[0] + sx
                     assert 0 <= idx < total
-                    if not vec:  # border-rule reads land on valid data
+                    if not vec:  # border-rule and row-source reads land on valid data
                         be0 = min(bs[0], ds[0] - (blk % nb[0]) * bs[0])
                         be1 = min(bs[1], ds[1] - (blk // nb[0]) * bs[1])
                         assert sx < be0 and sy < be1, "read of padding at %s" % ((b, x, y),)
@@ -123,11 +147,15 @@ def test_focal_reads_in_bounds(ds, bs, h0, h1, tile):
                 for ry in range(ty // (256 // twx)):
                     base = (ly + ry * (256 // twx)) * sw + lx
                     assert base + 2 * h1 * sw + vx + 2 * pad - 1 < hh * sw
-            # per-cell loads use clamped coordinates
-            for lx, ly in itertools.product(range(tx), range(ty)):
-                x, y = x0 + lx, y0 + ly
-                cx, cy = (x, y) if (x < e0 and y < e1) else (e0 - 1, e1 - 1)
-                assert 0 <= (b * bs[1] + cy) * bs[0] + cx < total
+            # per-thread cell vectors (loads at the clamped row, stores of the valid cells)
+            for xv, ly in itertools.product(range(x0, x0 + tx, vx), range(ty)):
+                y = y0 + ly
+                yc = min(y, e1 - 1)
+                xl = xv if xv < e0 else ((e0 - 1) // vx) * vx
+                assert xl + vx <= bs[0] and 0 <= (b * bs[1] + yc) * bs[0] + xl + vx <= total
+                nvalid = (vx if e0 - xv >= vx else max(0, e0 - xv)) if y < e1 else 0
+                if nvalid:
+                    assert (b * bs[1] + y) * bs[0] + xv + nvalid <= total and y < bs[1]
 
 
 def test_border_rule_matches_oracle():
