@@ -124,7 +124,10 @@ int ma_release(ma_node_t *node);
 /* Runtime option: "form" (0 plain: one launch per group per block, 1 batched),
  * "graph" (replay launch sequences from hipGraphs), "focal_zonal" (fuse a focal
  * group into the zonal group that consumes it), "tile_tx"/"tile_ty"/"tile_vx",
- * "wg_per_cu", "profile" (time every launch). Returns 0 on success. */
+ * "wg_per_cu", "profile" (time every launch), "batch_tasks" (independent groups
+ * of one kernel in one evaluation share one launch through a device task
+ * table), "queue" (device chunk queue: 0 batched launches only, 1 always,
+ * 2 never). Returns 0 on success. */
 int ma_setOption(const char *key, long long value);
 long long ma_getOption(const char *key);
 /* Fusion groups and generated kernel sources of an evaluation, as JSON, without
@@ -144,6 +147,8 @@ double ma_lastEvalMs(void);
 /* Durations of the kernels launched since the last call while the "profile"
  * option is set (HIP events around each launch), as JSON by kernel name. */
 const char *ma_kernelTimes(void);
+/* Generated HIP source of a kernel this process has built (by kernel name), or NULL. */
+const char *ma_kernelSource(const char *name);
 /* Autotuner: evaluates the nodes once per candidate tile shape (width,
  * height, cells per thread: also the layout of the LDS halo tile), times each
  * launch with HIP events, and keeps the fastest shape per fused-group

@@ -229,6 +229,10 @@ int ma_setOption(const char* key, long long value) {
     else if (k == "wg_per_cu") {
       if (value < 0 || value > 64) throw std::runtime_error("setOption: wg_per_cu in [0, 64]");
       o.wg_per_cu = (int)value;
+    } else if (k == "batch_tasks") o.batch_tasks = value != 0;
+    else if (k == "queue") {
+      if (value < 0 || value > 2) throw std::runtime_error("setOption: queue in {0 batched only, 1 always, 2 never}");
+      o.queue = (int)value;
     } else throw std::runtime_error("setOption: unknown option " + k);
     return 0;
   }, -1);
@@ -246,6 +250,8 @@ long long ma_getOption(const char* key) {
   if (k == "tile_ty") return o.tile.TY;
   if (k == "tile_vx") return o.tile.VX;
   if (k == "wg_per_cu") return o.wg_per_cu;
+  if (k == "batch_tasks") return o.batch_tasks;
+  if (k == "queue") return o.queue;
   return -1;
 }
 
@@ -311,6 +317,14 @@ const char* ma_kernelTimes(void) {
     }
     s << "}";
     g_json = s.str();
+    return g_json.c_str();
+  }, (const char*)nullptr);
+}
+
+const char* ma_kernelSource(const char* name) {
+  return guard([&] {
+    g_json = R().kernelSource(name ? name : "");
+    if (g_json.empty()) throw std::runtime_error("kernelSource: no kernel of that name was built");
     return g_json.c_str();
   }, (const char*)nullptr);
 }

@@ -155,8 +155,9 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     for (Node* p : n->prev) up(p);
   };
   for (Node* s : staged) up(s);
-  for (Node* n : halo)
-    if (n->pattern() == FOCAL) throw std::runtime_error("codegen: focal node feeding a focal node in one group");
+  for (Node* n : halo)  // inputs are materialised rasters, whatever produced them
+    if (n->pattern() == FOCAL && !in_set.count(n))
+      throw std::runtime_error("codegen: focal node feeding a focal node in one group");
 
   ks.form = focal.empty() ? Form::Local : Form::Focal;
   for (Node* f : focal) { ks.H0 = std::max(ks.H0, f->halo(0)); ks.H1 = std::max(ks.H1, f->halo(1)); }
@@ -196,7 +197,27 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     ks.params.push_back(p);
   }
   for (Node* n : d2_outs) { Param p; p.kind = Param::OutD2; p.node = n; p.dt = n->meta.dt; ks.params.push_back(p); }
-  size_t part_bytes = 0, result_bytes = 0;
+  // layout of P::Args: members in parameter order at their natural alignment
+  // (In<T> = {const T* p; T v; int fixed}, aligned to 8)
+  {
+    size_t off = 0, al_max = 1;
+    for (Param& p : ks.params) {
+      size_t sz = 8, al = 8;
+      if (p.kind == Param::InD2) sz = sizeof_dt(p.dt) == 8 ? 24 : 16;
+      if (p.kind == Param::InD0) sz = al = (size_t)sizeof_dt(p.dt);
+      off = (off + al - 1) / al * al;
+      p.off = off;
+      p.size = sz;
+      off += sz;
+      al_max = std::max(al_max, al);
+    }
+    if (ks.params.empty()) { off = 4; al_max = 4; }  // Args { int unused; }
+    ks.args_size = (off + al_max - 1) / al_max * al_max;
+    const size_t lal = std::max<size_t>(al_max, 4);
+    ks.task_launch_off = (ks.args_size + 3) / 4 * 4;
+    ks.task_size = (ks.task_launch_off + 13 * 4 + lal - 1) / lal * lal;
+  }
+  size_t part_bytes = 0;
   std::vector<size_t> part_off;
   for (Node* z : zonal) {
     ZonalOut zo{};
@@ -206,9 +227,8 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
       zo.red = z->op;
       zo.acc_dt = acc_type(z->op, zo.dt);
       part_off.push_back(part_bytes);
+      zo.result_off = part_bytes;
       part_bytes += 8;
-      zo.result_off = result_bytes;
-      result_bytes += 8;
     } else {
       if (!(zo.dt == F32 || zo.dt == S32 || zo.dt == U32 || zo.dt == U8 || zo.dt == U16 || zo.dt == S8 || zo.dt == S16))
         throw std::runtime_error("codegen: per-zone statistics support 8/16/32-bit data");
@@ -216,16 +236,12 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
       zo.acc_dt = F64;
       zo.nzones = z->nzones;
       part_off.push_back(part_bytes);
+      zo.result_off = part_bytes;
       part_bytes += (size_t)z->nzones * 24;
-      zo.result_off = result_bytes;
-      result_bytes += (size_t)z->nzones * 24;
     }
     ks.zouts.push_back(zo);
   }
   ks.part_bytes = part_bytes;
-  ks.result_bytes = result_bytes;
-  if (!zonal.empty()) { Param p; p.kind = Param::Partials; ks.params.push_back(p); }
-  { Param p; p.kind = Param::Launch; ks.params.push_back(p); }
 
   // the kernel is named after the hash of its own source (with the name
   // replaced by a placeholder), which is also the compile-cache key
@@ -246,6 +262,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     if (p.kind == Param::InD0) s << "    " << CT(p.dt) << " s" << LN(p.node) << ";\n";
     if (p.kind == Param::OutD2) s << "    " << CT(p.dt) << "* __restrict__ o" << LN(p.node) << ";\n";
   }
+  if (ks.params.empty()) s << "    int unused;\n";
   s << "  };\n";
   s << "  struct Smem {\n";
   for (Node* st : staged) s << "    alignas(16) " << CT(st->meta.dt) << " sh" << LN(st) << "[" << SW * HH << "];\n";
@@ -421,31 +438,40 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
         << V(z.node->prev[1]) << ");\n";
   }
   s << "    (void)a; (void)k; (void)x; (void)y; (void)ok; (void)W; (void)C; (void)v; (void)acc; (void)out;\n  }\n";
-  s << "};\n}  // namespace\n\n";
-
-  // main kernel
-  s << "extern \"C\" __global__ void __launch_bounds__(256) " << ks.name << "(";
-  bool first = true;
-  auto comma = [&]() { if (!first) s << ", "; first = false; };
-  for (const Param& p : ks.params) {
-    comma();
-    switch (p.kind) {
-      case Param::InD2: s << "mx::In<" << CT(p.dt) << "> i" << LN(p.node); break;
-      case Param::InD0: s << CT(p.dt) << " s" << LN(p.node); break;
-      case Param::OutD2: s << CT(p.dt) << "* o" << LN(p.node); break;
-      case Param::Partials: s << "char* zpart"; break;
-      case Param::Launch: s << "mx::Launch L"; break;
+  // flush(): writes one chunk's partial record and restarts the accumulators
+  s << "  static constexpr bool ZONAL = " << (zonal.empty() ? "false" : "true") << ";\n";
+  s << "  static constexpr unsigned PART_BYTES = " << part_bytes << "u;\n";
+  s << "  __device__ static __forceinline__ void flush(Acc& acc, char* my) {\n";
+  if (!zonal.empty()) s << "    __shared__ mx::u64 red_scratch[4];\n";
+  for (size_t j = 0; j < ks.zouts.size(); j++) {
+    const ZonalOut& z = ks.zouts[j];
+    if (z.node->kind == Kind::ZonalReduc) {
+      std::string A = CT(z.acc_dt);
+      s << "    mx::block_reduce_store<" << red_op(z.red) << ", " << A << ">(acc.r" << j << ", (" << A << "*)(my + "
+        << part_off[j] << "), (" << A << "*)red_scratch);\n";
+      s << "    acc.r" << j << " = " << lit(reduction_neutral(z.red, z.acc_dt)) << ";\n";
+    } else {
+      s << "    mx::zone_flush<" << CT(z.dt) << ", " << z.nzones << ">(acc.z" << j << ", (mx::ZonePart<" << CT(z.dt)
+        << ">*)(my + " << part_off[j] << "));\n";
+      s << "    __syncthreads();\n    mx::zone_init<" << CT(z.dt) << ", " << z.nzones << ">(acc.z" << j
+        << ");\n    __syncthreads();\n";
     }
   }
-  s << ") {\n";
-  s << "  P::Args a = {";
-  first = true;
-  for (const Param& p : ks.params) {
-    if (p.kind == Param::InD2) { comma(); s << "i" << LN(p.node); }
-    if (p.kind == Param::InD0) { comma(); s << "s" << LN(p.node); }
-    if (p.kind == Param::OutD2) { comma(); s << "o" << LN(p.node); }
-  }
+  s << "    (void)acc; (void)my;\n  }\n";
   s << "};\n";
+  // the host packs P::Args and TaskRec byte by byte: pin the layout
+  s << "static_assert(sizeof(P::Args) == " << ks.args_size << ", \"Args layout\");\n";
+  for (const Param& p : ks.params)
+    s << "static_assert(__builtin_offsetof(P::Args, " << (p.kind == Param::InD2 ? "i" : p.kind == Param::InD0 ? "s" : "o")
+      << LN(p.node) << ") == " << p.off << ", \"Args layout\");\n";
+  s << "static_assert(sizeof(mx::TaskRec<P::Args>) == " << ks.task_size << " && __builtin_offsetof(mx::TaskRec<P::Args>, L) == "
+    << ks.task_launch_off << ", \"TaskRec layout\");\n";
+  s << "}  // namespace\n\n";
+
+  // main kernel
+  s << "extern \"C\" __global__ void __launch_bounds__(256) " << ks.name
+    << "(P::Args a0, mx::Launch L0, const mx::TaskRec<P::Args>* tasks, const int* starts, int ntasks, int nchunks, "
+       "mx::u32* queue, char* zpart) {\n";
   s << "  P::Acc acc;\n";
   for (size_t j = 0; j < ks.zouts.size(); j++) {
     const ZonalOut& z = ks.zouts[j];
@@ -462,42 +488,33 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     }
   }
   if (!zonal.empty()) s << "  __syncthreads();\n";
-  if (ks.form == Form::Focal) {
-    s << "  __shared__ P::Smem sm;\n";
-    s << "  mx::focal_body<P, " << tile.TX << ", " << tile.TY << ", " << tile.VX << ">(a, L, acc, sm);\n";
-  } else {
-    s << "  mx::local_body<P, " << tile.TX << ", " << tile.TY << ", " << tile.VX << ">(a, L, acc);\n";
-  }
-  if (!zonal.empty()) {
-    s << "  __shared__ mx::u64 red_scratch[4];\n";
-    s << "  char* my = zpart + (size_t)(L.part0 + blockIdx.x) * " << part_bytes << "u;\n";
-    for (size_t j = 0; j < ks.zouts.size(); j++) {
-      const ZonalOut& z = ks.zouts[j];
-      if (z.node->kind == Kind::ZonalReduc) {
-        std::string A = CT(z.acc_dt);
-        s << "  mx::block_reduce_store<" << red_op(z.red) << ", " << A << ">(acc.r" << j << ", (" << A << "*)(my + " << part_off[j]
-          << "), (" << A << "*)red_scratch);\n";
-      } else {
-        s << "  mx::zone_flush<" << CT(z.dt) << ", " << z.nzones << ">(acc.z" << j << ", (mx::ZonePart<" << CT(z.dt)
-          << ">*)(my + " << part_off[j] << "));\n";
-      }
-    }
-  }
+  s << "  __shared__ P::Smem sm;\n  __shared__ int slot;\n";
+  s << "  mx::run_chunks<P, " << tile.TX << ", " << tile.TY << ", " << tile.VX << ", "
+    << (ks.form == Form::Focal ? "true" : "false") << ">(a0, L0, tasks, starts, ntasks, nchunks, queue, zpart, acc, sm, &slot);\n";
   s << "}\n";
 
-  // finalisation kernel: folds the partials of every work-group in a fixed order
+  // fold kernel: folds partial records in index order. Work-group (f, t) folds
+  // slice f of F of task t's records [base, base+n) of `src` into record
+  // t*F + f of `dst`: task t's chunks (tasks: from the table; stride == 0: one
+  // task of n_single chunks) or, with stride > 0, records [t*stride, (t+1)*stride).
   if (!zonal.empty()) {
     s << "\nextern \"C\" __global__ void __launch_bounds__(256) " << ks.fin_name
-      << "(const char* zpart, int nparts, char* res) {\n";
+      << "(const char* src, char* dst, const mx::TaskRec<P::Args>* tasks, const int* starts, int n_single, int stride, int F) {\n";
     s << "  __shared__ mx::u64 red_scratch[4];\n";
+    s << "  const int t = blockIdx.y, f = blockIdx.x;\n";
+    s << "  int base = 0, n = n_single;\n";
+    s << "  if (tasks) { base = tasks[t].L.part0; n = starts[t + 1] - starts[t]; }\n";
+    s << "  else if (stride > 0) { base = t * stride; n = stride; }\n";
+    s << "  const int b0 = base + (int)((long long)n * f / F), b1 = base + (int)((long long)n * (f + 1) / F);\n";
+    s << "  char* out = dst + (size_t)(t * F + f) * " << part_bytes << "u;\n";
     for (size_t j = 0; j < ks.zouts.size(); j++) {
       const ZonalOut& z = ks.zouts[j];
       if (z.node->kind == Kind::ZonalReduc) {
         std::string A = CT(z.acc_dt);
         s << "  {\n    " << A << " r = " << lit(reduction_neutral(z.red, z.acc_dt)) << ";\n";
-        s << "    for (int p = threadIdx.x; p < nparts; p += 256) r = " << red_op(z.red) << "::f(r, *(const " << A
-          << "*)(zpart + (size_t)p * " << part_bytes << "u + " << part_off[j] << "));\n";
-        s << "    mx::block_reduce_store<" << red_op(z.red) << ", " << A << ">(r, (" << A << "*)(res + " << z.result_off
+        s << "    for (int p = b0 + threadIdx.x; p < b1; p += 256) r = " << red_op(z.red) << "::f(r, *(const " << A
+          << "*)(src + (size_t)p * " << part_bytes << "u + " << part_off[j] << "));\n";
+        s << "    mx::block_reduce_store<" << red_op(z.red) << ", " << A << ">(r, (" << A << "*)(out + " << part_off[j]
           << "), (" << A << "*)red_scratch);\n  }\n";
       } else {
         std::string T = CT(z.dt);
@@ -505,12 +522,12 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
         s << "  for (int zi = threadIdx.x; zi < " << z.nzones << "; zi += 256) {\n";
         s << "    double sum = 0.0; " << M << " mn = mx::mm_highest<" << T << ">(), mxv = mx::mm_lowest<" << T
           << ">(); mx::u32 cnt = 0u;\n";
-        s << "    for (int p = 0; p < nparts; p++) {\n";
-        s << "      const mx::ZonePart<" << T << ">& q = ((const mx::ZonePart<" << T << ">*)(zpart + (size_t)p * "
+        s << "    for (int p = b0; p < b1; p++) {\n";
+        s << "      const mx::ZonePart<" << T << ">& q = ((const mx::ZonePart<" << T << ">*)(src + (size_t)p * "
           << part_bytes << "u + " << part_off[j] << "))[zi];\n";
         s << "      sum += q.sum; mn = mx::OpMin::f(mn, q.mn); mxv = mx::OpMax::f(mxv, q.mx); cnt += q.cnt;\n    }\n";
         s << "    mx::ZonePart<" << T << "> r; r.sum = sum; r.mn = mn; r.mx = mxv; r.cnt = cnt; r.pad = 0u;\n";
-        s << "    ((mx::ZonePart<" << T << ">*)(res + " << z.result_off << "))[zi] = r;\n  }\n";
+        s << "    ((mx::ZonePart<" << T << ">*)(out + " << part_off[j] << "))[zi] = r;\n  }\n";
       }
     }
     s << "}\n";

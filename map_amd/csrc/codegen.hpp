@@ -15,11 +15,12 @@ namespace mapx {
 
 enum class Form : int { Local = 0, Focal = 1 };
 
-// One kernel parameter (in declaration order).
+// One member of the kernel's argument struct (P::Args), in declaration order.
 struct Param {
-  enum Kind : int { InD2, InD0, OutD2, Partials, Launch } kind;
-  Node* node = nullptr;  // InD2/InD0/OutD2
+  enum Kind : int { InD2, InD0, OutD2 } kind;
+  Node* node = nullptr;
   int dt = 0;
+  size_t off = 0, size = 0;  // byte offset and size inside P::Args
 };
 
 // A zonal output of the group.
@@ -44,13 +45,20 @@ struct KernelSpec {
   TileShape tile;
   int H0 = 0, H1 = 0;
   std::vector<Param> params;
+  size_t args_size = 0;           // sizeof(P::Args)
+  size_t task_size = 0;           // sizeof(mx::TaskRec<P::Args>)
+  size_t task_launch_off = 0;     // offset of TaskRec::L
   std::vector<Node*> d2_outs;     // materialised D2 outputs, in parameter order
   std::vector<ZonalOut> zouts;
-  size_t part_bytes = 0;          // bytes of one work-group's partial record
-  size_t result_bytes = 0;        // bytes of the finalised results
+  size_t part_bytes = 0;          // bytes of one chunk's partial record (= one task's result record)
   std::string signature;          // cache key (group signature + tile shape)
 };
 
+// Kernel parameters (both kernels are extern "C", launched with these):
+//   main: (P::Args a0, mx::Launch L0, const mx::TaskRec<P::Args>* tasks,
+//          const int* starts, int ntasks, int nchunks, mx::u32* queue, char* zpart)
+//   fold: (const char* src, char* dst, const mx::TaskRec<P::Args>* tasks,
+//          const int* starts, int n_single, int stride, int F)
 // Builds the kernel of group `g`. `d2_outs` are the D2 nodes the group must
 // write to HBM (group outputs consumed later, written to files or requested).
 KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, TileShape tile);

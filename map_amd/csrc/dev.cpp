@@ -40,7 +40,7 @@ struct State {
   bool recording = false;
   struct Rec {
     hipFunction_t f;
-    unsigned grid, block;
+    unsigned grid, grid_y, block;
     std::vector<std::vector<char>> args;
   };
   std::vector<Rec> pending;
@@ -219,13 +219,15 @@ Function function(const std::string& source, const std::string& name) {
   return r;
 }
 
-void launch(const Function& f, unsigned grid, unsigned block, void** params, const size_t* sizes, int nparams) {
+void launch(const Function& f, unsigned grid, unsigned block, void** params, const size_t* sizes, int nparams,
+            unsigned grid_y) {
   State& s = S();
-  if (grid == 0) return;
+  if (grid == 0 || grid_y == 0) return;
   if (s.recording) {
     State::Rec r;
     r.f = (hipFunction_t)f.fn;
     r.grid = grid;
+    r.grid_y = grid_y;
     r.block = block;
     for (int i = 0; i < nparams; i++)
       r.args.emplace_back((const char*)params[i], (const char*)params[i] + sizes[i]);
@@ -241,7 +243,7 @@ void launch(const Function& f, unsigned grid, unsigned block, void** params, con
     ev.name = f.name;
     check(hipEventRecord(ev.a, s.stream), "hipEventRecord");
   }
-  check(hipModuleLaunchKernel((hipFunction_t)f.fn, grid, 1, 1, block, 1, 1, 0, s.stream, params, nullptr),
+  check(hipModuleLaunchKernel((hipFunction_t)f.fn, grid, grid_y, 1, block, 1, 1, 0, s.stream, params, nullptr),
         "hipModuleLaunchKernel");
   if (s.profile) {
     check(hipEventRecord(ev.b, s.stream), "hipEventRecord");
@@ -291,6 +293,12 @@ void record_begin() {
 bool recording() { return S().recording; }
 size_t graph_cache_size() { return S().graphs.size(); }
 
+void graph_clear() {
+  State& s = S();
+  for (auto& g : s.graphs) (void)hipGraphExecDestroy(g.second);
+  s.graphs.clear();
+}
+
 void record_flush() {
   State& s = S();
   if (!s.recording) return;
@@ -301,6 +309,7 @@ void record_flush() {
   for (auto& r : s.pending) {
     key.append((const char*)&r.f, sizeof r.f);
     key.append((const char*)&r.grid, sizeof r.grid);
+    key.append((const char*)&r.grid_y, sizeof r.grid_y);
     key.append((const char*)&r.block, sizeof r.block);
     for (auto& a : r.args) { key.append(a.data(), a.size()); key.push_back('\x1f'); }
     key.push_back('\x1e');
@@ -313,7 +322,7 @@ void record_flush() {
     for (auto& r : s.pending) {
       std::vector<void*> ptrs;
       for (auto& a : r.args) ptrs.push_back((void*)a.data());
-      hipError_t e = hipModuleLaunchKernel(r.f, r.grid, 1, 1, r.block, 1, 1, 0, s.stream, ptrs.data(), nullptr);
+      hipError_t e = hipModuleLaunchKernel(r.f, r.grid, r.grid_y, 1, r.block, 1, 1, 0, s.stream, ptrs.data(), nullptr);
       if (e != hipSuccess) {
         hipGraph_t g;
         (void)hipStreamEndCapture(s.stream, &g);

@@ -50,7 +50,8 @@ Function function(const std::string& source, const std::string& name);
 
 // Kernel launch on the compute stream. `params[i]` points at argument i of
 // `sizes[i]` bytes (the sizes let recorded launches copy their arguments).
-void launch(const Function& f, unsigned grid, unsigned block, void** params, const size_t* sizes, int nparams);
+void launch(const Function& f, unsigned grid, unsigned block, void** params, const size_t* sizes, int nparams,
+            unsigned grid_y = 1);
 
 // Launch recording for hipGraph replay: while recording, launches are queued;
 // flush() replays an identical earlier sequence from its instantiated graph,
@@ -59,6 +60,7 @@ void record_begin();
 void record_flush();
 bool recording();
 size_t graph_cache_size();
+void graph_clear();  // destroys the instantiated graphs (call with the stream idle)
 
 // Device timing of the compute stream with HIP events.
 struct Timer {

@@ -1,5 +1,7 @@
 #include "fuser.hpp"
 #include <algorithm>
+#include <iterator>
+#include <string>
 #include <sstream>
 #include <stdexcept>
 
@@ -41,9 +43,14 @@ const Rule kFlat[] = {
 };
 }  // namespace
 
+// `top` is the producer's pattern on the edge, `bot` the consumer's (how the
+// consuming nodes read the producer's output).
 bool can_pipe_fuse(int top, int bot, bool focal_zonal) {
-  if (focal_zonal && pattern_is(top, FOCAL) && pattern_is(bot, ZONAL) && !pattern_is(top, ZONAL) &&
-      !pattern_is(top, RADIAL) && !pattern_is(top, SPREAD))
+  // extension: a focal producer read cell by cell (not through a halo) by a
+  // zonal consumer fuses into it
+  if (focal_zonal && pattern_is(top, FOCAL) && pattern_is(bot, ZONAL) && !pattern_is(bot, FOCAL) &&
+      !pattern_is(top, ZONAL) && !pattern_is(top, RADIAL) && !pattern_is(top, SPREAD) &&
+      !pattern_is(bot, RADIAL) && !pattern_is(bot, SPREAD))
     return true;
   for (const Rule& r : kPipe)
     if (pattern_is(top, r.a) && pattern_is(bot, r.b)) return r.r;
@@ -175,8 +182,19 @@ bool Fuser::reaches(const Group* from, const Group* to) const {
   return false;
 }
 
+// Groups over rasters of different geometry never share a kernel (one launch
+// walks one block grid). The reference has no such check: its evaluations
+// hold one raster geometry, while here independent scripts over rasters of
+// different sizes may be evaluated together (and share D0 constants).
+static bool same_geometry(const Group* a, const Group* b) {
+  const Meta ma = a->meta(), mb = b->meta();
+  if (ma.nd != D2 || mb.nd != D2) return true;
+  return dim2_eq(ma.ds, mb.ds) && dim2_eq(ma.bs, mb.bs);
+}
+
 bool Fuser::canPipe(Group* top, Group* bot) {  // Fusioner.cpp:87-96
   if (top == bot) return false;
+  if (!same_geometry(top, bot)) return false;
   for (Group* n : top->next)
     if (n != bot && isNext(bot, n)) return false;  // a path top -> n -> ... -> bot: fusing makes a cycle
   return can_pipe_fuse(bot->prevPattern(top), top->nextPattern(bot), opt_.focal_zonal) &&
@@ -185,6 +203,7 @@ bool Fuser::canPipe(Group* top, Group* bot) {  // Fusioner.cpp:87-96
 
 bool Fuser::canFlat(Group* left, Group* right) {  // Fusioner.cpp:98-106
   if (left == right) return false;
+  if (!same_geometry(left, right)) return false;
   if (isNext(right, left) || isNext(left, right)) return false;
   return can_flat_fuse(left->pat, right->pat, opt_.focal_zonal);
 }
@@ -265,13 +284,35 @@ void Fuser::flatGently(Node* node) {  // Fusioner.cpp:280-309
   Group* ng = of_[node].front();
   if (ng->next.size() < 2) return;
   const auto& nx = nexts(node);
+  // candidates of one left group: the later next-nodes whose groups have its
+  // raster geometry (same_geometry rejects every other pair), in order
+  std::map<std::string, std::vector<size_t>> by_geom;
+  std::vector<std::string> key(nx.size());
+  for (size_t k = 0; k < nx.size(); k++) {
+    const Meta m = of_[nx[k]].front()->meta();
+    if (m.nd == D2) {
+      int v[4] = {m.ds.arr[0], m.ds.arr[1], m.bs.arr[0], m.bs.arr[1]};
+      key[k].assign((const char*)v, sizeof v);
+    }
+    by_geom[key[k]].push_back(k);
+  }
   size_t i = 0;
   while (i < nx.size()) {
+    const size_t li = i;
     Group* left = of_[nx[i++]].front();
     if (!free_or_local(left) || left == ng) continue;
-    size_t j = i;
-    while (j < nx.size()) {
-      Group* right = of_[nx[j++]].front();
+    // a D0 left group may pair with any geometry: scan all later nodes
+    const bool any = key[li].empty();
+    std::vector<size_t> cand;
+    if (any) for (size_t j = i; j < nx.size(); j++) cand.push_back(j);
+    else {
+      const auto& same = by_geom[key[li]];
+      const auto& d0 = by_geom[std::string()];
+      std::merge(std::upper_bound(same.begin(), same.end(), li), same.end(), std::upper_bound(d0.begin(), d0.end(), li),
+                 d0.end(), std::back_inserter(cand));
+    }
+    for (size_t j : cand) {
+      Group* right = of_[nx[j]].front();
       if (!free_or_local(right) || right == ng || right == left) continue;
       if (canFlat(left, right)) left = flatFuse(left, right);
     }
@@ -308,10 +349,14 @@ std::vector<Node*> Fuser::allNodes(Group* g) const {
 
 void Fuser::forwarding() {  // Fusioner.cpp:394-450, predicate: FREE nodes
   std::map<std::pair<Group*, Group*>, std::vector<Node*>> forward;
+  // after its first visit a FREE node is in the group of every next-node
+  // (groups do not change here), so later visits would add nothing
+  std::set<Node*> visited;
   for (size_t gi = 0; gi < groups_.size(); gi++) {
     Group* g = groups_[gi].get();
     for (Node* node : allNodes(g)) {
       if (node->pattern() != FREE) continue;
+      if (!visited.insert(node).second) continue;
       bool forwarded = false;
       for (Node* nn : nexts(node)) {
         for (Group* ng : std::vector<Group*>(of_[nn])) {
@@ -352,12 +397,19 @@ void Fuser::forwarding() {  // Fusioner.cpp:394-450, predicate: FREE nodes
 }
 
 void Fuser::linking() {  // Fusioner.cpp:452-509
-  for (auto& up : groups_) {
-    Group* g = up.get();
-    for (Node* n : g->nodes)
-      for (Node* nn : nexts(n))
-        for (Group* ng : of_[nn])
-          if (!contains(ng->nodes, n)) { g->addOutput(n); ng->addInput(n); }
+  // a node computed in group g and read in group ng (which does not compute
+  // it) becomes an output of g and an input of ng; visited per node, so a
+  // node computed in many groups (a forwarded constant) costs its edges once
+  std::unordered_map<Node*, std::vector<Group*>> in_nodes;
+  for (auto& up : groups_)
+    for (Node* n : up->nodes) in_nodes[n].push_back(up.get());
+  for (auto& e : in_nodes) {
+    Node* n = e.first;
+    const std::vector<Group*>& gs = e.second;
+    for (Node* nn : nexts(n))
+      for (Group* ng : of_[nn])
+        if (!contains(gs, ng))
+          for (Group* g : gs) { g->addOutput(n); ng->addInput(n); }
   }
   // D0 FREE nodes become inputs: better reuse of compiled kernels (Fusioner.cpp:494-508).
   for (auto& up : groups_) {
@@ -385,27 +437,28 @@ void Fuser::sorting() {  // Fusioner.cpp:511-543
   // Topological order: a group precedes every group that depends on it; ties
   // broken by the id of the last node. A stable topological sort replaces the
   // reference's std::sort with a partial-order comparator.
-  std::vector<Group*> order, ready;
-  std::map<Group*, int> indeg;
-  for (auto& g : groups_) indeg[g.get()] = (int)g->prev.size();
-  auto last_id = [](Group* g) {
+  std::unordered_map<Group*, int> indeg, last;
+  for (auto& g : groups_) {
+    indeg[g.get()] = (int)g->prev.size();
     int m = -1;
     for (Node* n : g->nodes) m = std::max(m, n->id);
     for (Node* n : g->outs) m = std::max(m, n->id);
-    return m;
-  };
-  for (auto& g : groups_) if (indeg[g.get()] == 0) ready.push_back(g.get());
-  while (!ready.empty()) {
-    auto it = std::min_element(ready.begin(), ready.end(), [&](Group* a, Group* b) { return last_id(a) < last_id(b); });
-    Group* g = *it;
-    ready.erase(it);
-    order.push_back(g);
-    for (Group* n : g->next) if (--indeg[n] == 0) ready.push_back(n);
+    last[g.get()] = m;
   }
-  if (order.size() != groups_.size()) throw std::runtime_error("fuser: group graph has a cycle");
+  // ready groups by (last node id, position): the earliest-finishing first
+  std::set<std::pair<int, size_t>> ready;
+  std::unordered_map<Group*, size_t> pos;
+  for (size_t i = 0; i < groups_.size(); i++) pos[groups_[i].get()] = i;
+  for (auto& g : groups_) if (indeg[g.get()] == 0) ready.insert({last[g.get()], pos[g.get()]});
   std::vector<std::unique_ptr<Group>> sorted;
-  for (Group* g : order)
-    for (auto& up : groups_) if (up.get() == g) { sorted.push_back(std::move(up)); break; }
+  while (!ready.empty()) {
+    const size_t i = ready.begin()->second;
+    ready.erase(ready.begin());
+    Group* g = groups_[i].get();
+    sorted.push_back(std::move(groups_[i]));
+    for (Group* n : g->next) if (--indeg[n] == 0) ready.insert({last[n], pos[n]});
+  }
+  if (sorted.size() != groups_.size()) throw std::runtime_error("fuser: group graph has a cycle");
   groups_.swap(sorted);
   int id = 0;
   for (auto& g : groups_) g->id = id++;

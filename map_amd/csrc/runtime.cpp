@@ -573,21 +573,9 @@ void Runtime::scalar_task(const Group& g) {
 }
 
 namespace {
-template <class T> struct HostIn { const T* p; T v; int fixed; };
-struct HostLaunch { int BS0, BS1, NB0, NB1, DS0, DS1, blk0, nblk, TB0, TB1, part0; };
-
-size_t in_size(int dt) {
-  switch (sizeof_dt(dt)) {
-    case 1: return sizeof(HostIn<uint8_t>); case 2: return sizeof(HostIn<uint16_t>);
-    case 4: return sizeof(HostIn<uint32_t>); default: return sizeof(HostIn<uint64_t>);
-  }
-}
-// Packs an In<T> for a D2 input: pointer, value and fixed flag at their C offsets.
-std::vector<char> pack_in(int dt, const void* p) {
-  std::vector<char> b(in_size(dt), 0);
-  std::memcpy(b.data(), &p, sizeof p);
-  return b;  // value 0, fixed 0: inputs are always materialised rasters
-}
+// mx::Launch of the device templates
+struct HostLaunch { int BS0, BS1, NB0, NB1, DS0, DS1, blk0, nblk, TB0, TB1, CH, CPB, part0; };
+static_assert(sizeof(HostLaunch) == 52, "mx::Launch layout");
 
 uint32_t morton2(uint32_t x, uint32_t y) {
   auto spread = [](uint32_t v) {
@@ -598,14 +586,31 @@ uint32_t morton2(uint32_t x, uint32_t y) {
   };
   return spread(x) | (spread(y) << 1);
 }
+
+// Cells per chunk of a zonal group: one partial record per chunk.
+constexpr int kZonalChunkCells = 65536;
+// Tiles per chunk of other groups when a device queue hands out the chunks.
+constexpr int kQueueChunkTiles = 8;
 }  // namespace
 
-void Runtime::run_group(const Group& g, const std::vector<Node*>& targets,
-                        const std::vector<std::unique_ptr<Group>>& all) {
-  (void)all;
-  cnt.groups++;
-  if (g.numdim() == D0) { scalar_task(g); return; }
+// One fused group over its raster: the unit a launch runs, alone or in a batch.
+struct Runtime::Task {
+  const Group* g = nullptr;
+  KernelEntry* ke = nullptr;
+  std::vector<char> args;  // P::Args
+  HostLaunch L{};
+  int tiles = 0;           // tiles per block
+  int nchunks = 0;
 
+  // chunk split: zonal groups keep CH fixed (it decides their partial records)
+  void chunking(bool queue) {
+    if (ke->spec.zouts.empty()) L.CH = std::min(tiles, queue ? kQueueChunkTiles : 1);
+    L.CPB = (tiles + L.CH - 1) / L.CH;
+    nchunks = L.nblk * L.CPB;
+  }
+};
+
+bool Runtime::prepare_task(const Group& g, const std::vector<Node*>& targets, Task& t) {
   std::vector<Node*> outs = d2_outputs(g, targets);
   std::vector<Node*> kouts;
   for (Node* n : outs)
@@ -617,118 +622,239 @@ void Runtime::run_group(const Group& g, const std::vector<Node*>& targets,
     if (n->meta.nd == D2) ensure_input(n);
     else if (!n->evaluated) fail("D0 input of node " + std::to_string(n->id) + " is not evaluated");
   }
+  if (kouts.empty() && !has_zonal) return false;
 
-  if (!kouts.empty() || has_zonal) {
-    const Meta gm = g.meta();
-    for (Node* n : kouts) {
-      if (!dim2_eq(n->meta.ds, gm.ds) || !dim2_eq(n->meta.bs, gm.bs))
-        fail("group outputs differ in raster geometry");
-      void* d = store.alloc(rasterBytes(n->meta));
-      resident_[n] = d;
-    }
-    KernelEntry& ke = kernel_for(cur_plan_, g, kouts);
-    const KernelSpec& ks = ke.spec;
-    const TileShape tile = ks.tile;
-    if (ks.form == Form::Focal) check_focal_geometry(gm, ks.H0, ks.H1);
-    dev::Function fn;
-    fn.fn = ke.fn;
-    fn.name = ks.name;
-    HostTimer ht_launch(cnt.ms_launch);
-
-    const int BS0 = gm.bs.arr[0], BS1 = gm.bs.arr[1], NB0 = gm.nb(0), NB1 = gm.nb(1);
-    const int TB0 = (BS0 + tile.TX - 1) / tile.TX, TB1 = (BS1 + tile.TY - 1) / tile.TY;
-    // persistent grid: every work-group resident at once (no tail wave), a
-    // multiple of 8 for the XCD-contiguous work split
-    const int per_cu = opt.wg_per_cu > 0 ? std::min(opt.wg_per_cu, ke.occ) : ke.occ;
-    const int gmax = std::max(8, (dev::num_cus() * per_cu) & ~7);
-    auto grid_for = [&](long work) {
-      long gsz = std::min<long>(work, gmax);
-      return (int)((gsz + 7) & ~7L);
-    };
-    std::vector<std::pair<int, int>> launches;  // (blk0, nblk)
-    if (opt.form == FORM_PLAIN) {
-      std::vector<std::pair<uint32_t, int>> order;
-      for (int b = 0; b < NB0 * NB1; b++) order.push_back({morton2(b % NB0, b / NB0), b});
-      std::sort(order.begin(), order.end());
-      for (auto& o : order) launches.push_back({o.second, 1});
+  const Meta gm = g.meta();
+  for (Node* n : kouts) {
+    if (!dim2_eq(n->meta.ds, gm.ds) || !dim2_eq(n->meta.bs, gm.bs))
+      fail("group outputs differ in raster geometry");
+    resident_[n] = store.alloc(rasterBytes(n->meta));
+  }
+  KernelEntry& ke = kernel_for(cur_plan_, g, kouts);
+  const KernelSpec& ks = ke.spec;
+  if (ks.form == Form::Focal) check_focal_geometry(gm, ks.H0, ks.H1);
+  t.g = &g;
+  t.ke = &ke;
+  t.args.assign(ks.args_size, 0);
+  for (const Param& p : ks.params) {
+    if (p.kind == Param::InD2 || p.kind == Param::OutD2) {
+      // In<T>: pointer, value 0, fixed 0 (inputs are materialised rasters)
+      void* d = resident_.at(p.node);
+      std::memcpy(t.args.data() + p.off, &d, sizeof d);
     } else {
-      launches.push_back({0, NB0 * NB1});
-    }
-    long total_parts = 0;
-    for (auto& l : launches) total_parts += grid_for((long)l.second * TB0 * TB1);
-    void* parts = nullptr;
-    void* res = nullptr;
-    if (has_zonal) {
-      parts = store.alloc(std::max<size_t>(1, (size_t)total_parts * ks.part_bytes));
-      res = store.alloc(std::max<size_t>(1, ks.result_bytes));
-    }
-
-    // argument storage in parameter order
-    std::vector<std::vector<char>> store_args;
-    int launch_idx = -1;
-    for (const Param& p : ks.params) {
-      switch (p.kind) {
-        case Param::InD2: store_args.push_back(pack_in(p.dt, resident_.at(p.node))); break;
-        case Param::InD0: {
-          Variant v = variant_convert(p.node->value, p.dt);
-          std::vector<char> b(sizeof_dt(p.dt));
-          std::memcpy(b.data(), &v.u, b.size());
-          store_args.push_back(b);
-          break;
-        }
-        case Param::OutD2: {
-          void* d = resident_.at(p.node);
-          store_args.emplace_back((char*)&d, (char*)&d + sizeof d);
-          break;
-        }
-        case Param::Partials: store_args.emplace_back((char*)&parts, (char*)&parts + sizeof parts); break;
-        case Param::Launch:
-          launch_idx = (int)store_args.size();
-          store_args.emplace_back(sizeof(HostLaunch), 0);
-          break;
-      }
-    }
-    std::vector<void*> ptrs;
-    std::vector<size_t> sizes;
-    for (auto& a : store_args) { ptrs.push_back(a.data()); sizes.push_back(a.size()); }
-    long part0 = 0;
-    for (auto& l : launches) {
-      HostLaunch L{BS0, BS1, NB0, NB1, gm.ds.arr[0], gm.ds.arr[1], l.first, l.second, TB0, TB1, (int)part0};
-      std::memcpy(store_args[launch_idx].data(), &L, sizeof L);
-      const int grid = grid_for((long)l.second * TB0 * TB1);
-      dev::launch(fn, (unsigned)grid, 256, ptrs.data(), sizes.data(), (int)ptrs.size());
-      cnt.launches++;
-      part0 += grid;
-    }
-    if (has_zonal) {
-      dev::Function fin;
-      fin.fn = ke.fin;
-      fin.name = ks.fin_name;
-      int np = (int)total_parts;
-      void* fargs[3] = {&parts, &np, &res};
-      size_t fsz[3] = {sizeof parts, sizeof np, sizeof res};
-      dev::launch(fin, 1, 256, fargs, fsz, 3);
-      cnt.launches++;
-      PauseRecording pause;
-      std::vector<unsigned char> host(ks.result_bytes);
-      dev::d2h(host.data(), res, ks.result_bytes);
-      dev::sync();
-      for (const ZonalOut& z : ks.zouts) {
-        if (z.node->kind == Kind::ZonalReduc) {
-          Variant a = variant_zero(z.acc_dt);
-          std::memcpy(&a.u, host.data() + z.result_off, sizeof_dt(z.acc_dt));
-          z.node->value = variant_convert(a, z.node->meta.dt);
-        } else {
-          z.node->table.assign(host.begin() + z.result_off, host.begin() + z.result_off + (size_t)z.nzones * 24);
-        }
-        z.node->evaluated = true;
-      }
-      store.release(parts);
-      store.release(res);
+      Variant v = variant_convert(p.node->value, p.dt);
+      std::memcpy(t.args.data() + p.off, &v.u, (size_t)sizeof_dt(p.dt));
     }
   }
-  for (Node* n : g.outs)
-    if (n->kind == Kind::Write) write_file(n);
+  const TileShape tile = ks.tile;
+  const int BS0 = gm.bs.arr[0], BS1 = gm.bs.arr[1], NB0 = gm.nb(0), NB1 = gm.nb(1);
+  const int TB0 = (BS0 + tile.TX - 1) / tile.TX, TB1 = (BS1 + tile.TY - 1) / tile.TY;
+  t.tiles = TB0 * TB1;
+  t.L = HostLaunch{BS0, BS1, NB0, NB1, gm.ds.arr[0], gm.ds.arr[1], 0, NB0 * NB1, TB0, TB1, 1, 1, 0};
+  if (!ks.zouts.empty()) t.L.CH = std::max(1, std::min(t.tiles, kZonalChunkCells / (tile.TX * tile.TY)));
+  return true;
+}
+
+void* Runtime::device_table(const std::string& bytes) {
+  auto it = tables_.find(bytes);
+  if (it != tables_.end()) return it->second;
+  if (table_bytes_ + bytes.size() > ((size_t)256 << 20)) {
+    // tables are immutable while anything may read them: drop them all at once
+    PauseRecording pause;
+    dev::sync();
+    dev::graph_clear();
+    for (auto& e : tables_) dev::free_device(e.second);
+    tables_.clear();
+    table_bytes_ = 0;
+  }
+  void* d = dev::malloc_device(std::max<size_t>(bytes.size(), 16));
+  auto ins = tables_.emplace(bytes, d).first;
+  dev::h2d(d, ins->first.data(), bytes.size());  // the key keeps the source bytes alive
+  table_bytes_ += bytes.size();
+  return d;
+}
+
+void* Runtime::queue() {
+  if (!queue_) {
+    queue_ = dev::malloc_device(256);
+    static const unsigned zero[64] = {0};
+    PauseRecording pause;
+    dev::h2d(queue_, zero, sizeof zero);
+    dev::sync();
+  }
+  return queue_;
+}
+
+// Folds the partial records of the tasks of one launch (or one task) in index
+// order and reads the zonal results back.
+void Runtime::fold_results(const std::vector<Task*>& batch, void* parts, void* tasks_dev, void* starts_dev, int maxn) {
+  const KernelEntry& ke = *batch[0]->ke;
+  const KernelSpec& ks = ke.spec;
+  const int T = (int)batch.size();
+  const size_t PB = ks.part_bytes;
+  dev::Function fin;
+  fin.fn = ke.fin;
+  fin.name = ks.fin_name;
+  const int F = std::max(1, std::min(256, (maxn + 63) / 64));
+  void* res = store.alloc(std::max<size_t>(1, (size_t)T * PB));
+  void* mid = F > 1 ? store.alloc((size_t)T * F * PB) : nullptr;
+  int n_single = T == 1 ? batch[0]->nchunks : 0;
+  {
+    void* dst = F > 1 ? mid : res;
+    int stride = 0;
+    void* a[7] = {&parts, &dst, &tasks_dev, &starts_dev, &n_single, &stride, (void*)&F};
+    size_t z[7] = {8, 8, 8, 8, 4, 4, 4};
+    dev::launch(fin, (unsigned)F, 256, a, z, 7, (unsigned)T);
+    cnt.launches++;
+  }
+  if (F > 1) {
+    void* nul = nullptr;
+    int zero = 0, one = 1, stride = F;
+    void* a[7] = {&mid, &res, &nul, &nul, &zero, &stride, &one};
+    size_t z[7] = {8, 8, 8, 8, 4, 4, 4};
+    dev::launch(fin, 1, 256, a, z, 7, (unsigned)T);
+    cnt.launches++;
+  }
+  PauseRecording pause;
+  std::vector<unsigned char> host((size_t)T * PB);
+  dev::d2h(host.data(), res, host.size());
+  dev::sync();
+  cnt.bytes_d2h += (long)host.size();
+  for (int t = 0; t < T; t++) {
+    const unsigned char* rec = host.data() + (size_t)t * PB;
+    for (const ZonalOut& z0 : ks.zouts) {
+      // the task's own node of this zonal output: same position in its group's kernel
+      Node* node = batch[t]->ke->spec.zouts[&z0 - ks.zouts.data()].node;
+      if (node->kind == Kind::ZonalReduc) {
+        Variant a = variant_zero(z0.acc_dt);
+        std::memcpy(&a.u, rec + z0.result_off, sizeof_dt(z0.acc_dt));
+        node->value = variant_convert(a, node->meta.dt);
+      } else {
+        node->table.assign(rec + z0.result_off, rec + z0.result_off + (size_t)z0.nzones * 24);
+      }
+      node->evaluated = true;
+    }
+  }
+  if (mid) store.release(mid);
+  store.release(res);
+}
+
+// Launches a batch of tasks of one kernel: per block (plain form), one task by
+// value, or many tasks from a device task table with a device chunk queue.
+void Runtime::run_batch(const std::vector<Task*>& batch) {
+  HostTimer ht_launch(cnt.ms_launch);
+  const KernelEntry& ke = *batch[0]->ke;
+  const KernelSpec& ks = ke.spec;
+  const bool zonal = !ks.zouts.empty();
+  dev::Function fn;
+  fn.fn = ke.fn;
+  fn.name = ks.name;
+  // persistent grid: every work-group resident at once (no tail wave), a
+  // multiple of 8 for the XCD-contiguous split
+  const int per_cu = opt.wg_per_cu > 0 ? std::min(opt.wg_per_cu, ke.occ) : ke.occ;
+  const int gmax = std::max(8, (dev::num_cus() * per_cu) & ~7);
+  auto grid_for = [&](long work) { return (unsigned)((std::min<long>(work, gmax) + 7) & ~7L); };
+  void* nul = nullptr;
+  auto launch = [&](std::vector<char>& args, HostLaunch& L, void* tasks, void* starts, int ntasks, int nchunks,
+                    void* queue, void* parts, unsigned grid) {
+    void* a[8] = {args.data(), &L, &tasks, &starts, &ntasks, &nchunks, &queue, &parts};
+    size_t z[8] = {args.size(), sizeof L, 8, 8, 4, 4, 8, 8};
+    dev::launch(fn, grid, 256, a, z, 8);
+    cnt.launches++;
+  };
+
+  if (opt.form == FORM_PLAIN) {
+    for (Task* t : batch) {
+      t->chunking(false);
+      void* parts = zonal ? store.alloc((size_t)t->nchunks * ks.part_bytes) : nullptr;
+      std::vector<std::pair<uint32_t, int>> order;
+      for (int b = 0; b < t->L.nblk; b++) order.push_back({morton2(b % t->L.NB0, b / t->L.NB0), b});
+      std::sort(order.begin(), order.end());
+      for (auto& o : order) {
+        HostLaunch L = t->L;
+        L.blk0 = o.second;
+        L.nblk = 1;
+        L.part0 = o.second * L.CPB;
+        launch(t->args, L, nul, nul, 1, L.CPB, nul, parts, grid_for(L.CPB));
+      }
+      if (zonal) {
+        fold_results({t}, parts, nullptr, nullptr, t->nchunks);
+        store.release(parts);
+      }
+    }
+    return;
+  }
+  if (batch.size() == 1) {
+    Task* t = batch[0];
+    const bool q = opt.queue == 1;
+    t->chunking(q);
+    void* parts = zonal ? store.alloc((size_t)t->nchunks * ks.part_bytes) : nullptr;
+    launch(t->args, t->L, nul, nul, 1, t->nchunks, q ? queue() : nul, parts, grid_for(t->nchunks));
+    if (zonal) {
+      fold_results(batch, parts, nullptr, nullptr, t->nchunks);
+      store.release(parts);
+    }
+    return;
+  }
+  // task table: TaskRec per task, then starts[ntasks + 1]
+  const bool q = opt.queue != 2;
+  const int T = (int)batch.size();
+  std::vector<int> starts(T + 1, 0);
+  int maxn = 0;
+  for (int i = 0; i < T; i++) {
+    batch[i]->chunking(q);
+    starts[i + 1] = starts[i] + batch[i]->nchunks;
+    maxn = std::max(maxn, batch[i]->nchunks);
+  }
+  const int total = starts[T];
+  std::string tab((size_t)T * ks.task_size + (size_t)(T + 1) * sizeof(int), '\0');
+  for (int i = 0; i < T; i++) {
+    HostLaunch L = batch[i]->L;
+    L.part0 = starts[i];
+    char* rec = &tab[(size_t)i * ks.task_size];
+    std::memcpy(rec, batch[i]->args.data(), ks.args_size);
+    std::memcpy(rec + ks.task_launch_off, &L, sizeof L);
+  }
+  std::memcpy(&tab[(size_t)T * ks.task_size], starts.data(), starts.size() * sizeof(int));
+  char* dtab = (char*)device_table(tab);
+  void* tasks_dev = dtab;
+  void* starts_dev = dtab + (size_t)T * ks.task_size;
+  void* parts = zonal ? store.alloc((size_t)total * ks.part_bytes) : nullptr;
+  HostLaunch L0{};
+  std::vector<char> a0(ks.args_size, 0);
+  launch(a0, L0, tasks_dev, starts_dev, T, total, q ? queue() : nul, parts, grid_for(total));
+  if (zonal) {
+    fold_results(batch, parts, tasks_dev, starts_dev, maxn);
+    store.release(parts);
+  }
+}
+
+void Runtime::run_level(const std::vector<const Group*>& level, const std::vector<Node*>& targets) {
+  std::vector<Task> tasks(level.size());
+  std::vector<Task*> order;
+  for (size_t i = 0; i < level.size(); i++) {
+    const Group& g = *level[i];
+    cnt.groups++;
+    if (g.numdim() == D0) { scalar_task(g); continue; }
+    if (prepare_task(g, targets, tasks[i])) order.push_back(&tasks[i]);
+  }
+  // tasks of one kernel form one batch (in order of first appearance)
+  std::vector<std::vector<Task*>> batches;
+  std::unordered_map<std::string, size_t> at;
+  for (Task* t : order) {
+    const bool one = opt.form == FORM_PLAIN || !opt.batch_tasks;
+    auto it = one ? at.end() : at.find(t->ke->spec.name);
+    if (it == at.end()) {
+      if (!one) at[t->ke->spec.name] = batches.size();
+      batches.push_back({t});
+    } else {
+      batches[it->second].push_back(t);
+    }
+  }
+  for (auto& b : batches) run_batch(b);
+  for (const Group* g : level)
+    for (Node* n : g->outs)
+      if (n->kind == Kind::Write) write_file(n);
 }
 
 std::string Runtime::plan_key(const std::vector<Node*>& list, const std::vector<Node*>& targets) const {
@@ -750,6 +876,7 @@ Runtime::KernelEntry& Runtime::kernel_for(Plan* plan, const Group& g, const std:
   KernelEntry e;
   e.spec = generate_kernel(g, kouts, tile_for(g));
   kernel_group_[e.spec.name] = tune_key(g);
+  kernel_src_[e.spec.name] = e.spec.source;
   dev::Function f = dev::function(e.spec.source, e.spec.name);
   e.fn = f.fn;
   e.occ = dev::occupancy(f, 256);
@@ -798,14 +925,30 @@ void Runtime::evaluate(const std::vector<Node*>& targets) {
   std::set<Node*> tset(targets.begin(), targets.end());
   for (Node* t : targets) if (t->kind == Kind::Write) tset.insert(t->prev[0]);
 
+  // levels: a group runs one level after the last group it depends on; the
+  // groups of one level are independent, and those sharing a kernel are batched
+  std::unordered_map<const Group*, int> lvl;
+  std::vector<std::vector<const Group*>> levels;
+  for (auto& g : groups) {
+    int l = 0;
+    for (Group* p : g->prev) {
+      auto it = lvl.find(p);
+      if (it != lvl.end()) l = std::max(l, it->second + 1);
+    }
+    lvl[g.get()] = l;
+    if ((int)levels.size() <= l) levels.resize(l + 1);
+    levels[l].push_back(g.get());
+  }
+
   if (opt.graph) dev::record_begin();
   try {
-    for (auto& g : groups) {
-      run_group(*g, targets, groups);
-      for (Node* n : g->ins) {
-        if (n->meta.nd != D2) continue;
-        if (--uses[n] == 0 && ext_refs(n) <= 0 && !tset.count(n) && n->kind != Kind::Read) drop(n);
-      }
+    for (auto& level : levels) {
+      run_level(level, targets);
+      for (const Group* g : level)
+        for (Node* n : g->ins) {
+          if (n->meta.nd != D2) continue;
+          if (--uses[n] == 0 && ext_refs(n) <= 0 && !tset.count(n) && n->kind != Kind::Read) drop(n);
+        }
     }
     if (dev::recording()) dev::record_flush();
   } catch (...) {
@@ -871,9 +1014,13 @@ std::string Runtime::plan(const std::vector<Node*>& targets) {
       for (Node* n : g->nodes) if (n->kind == Kind::ZonalReduc || n->kind == Kind::ZonalStats) has_zonal = true;
       if (!kouts.empty() || has_zonal) {
         KernelSpec ks;
-        {
+        try {
           HostTimer ht(cnt.ms_codegen);
           ks = generate_kernel(*g, kouts, tile_for(*g));
+        } catch (const std::exception& e) {
+          // the grouping is still reported; evaluation would fail with this error
+          j << ", \"error\": \"" << esc(e.what()) << "\"}";
+          continue;
         }
         j << ", \"kernel\": \"" << ks.name << "\", \"form\": " << (int)ks.form << ", \"halo\": [" << ks.H0 << ", " << ks.H1
           << "], \"source\": \"" << esc(ks.source) << "\"";
@@ -941,6 +1088,9 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   }
   rep << "]}";
   for (Node* t : targets) drop(t);
+  // cached plans hold kernels generated with the previous tile shapes
+  plans_.clear();
+  cur_plan_ = nullptr;
   if (const char* p = std::getenv("MAPX_TUNE_CACHE")) tune_save(p);
   return rep.str();
 }

@@ -29,6 +29,8 @@ struct Options {
   bool focal_zonal = false;   // fuse FOCAL -> ZONAL into one kernel
   bool code_fusion = true;
   int wg_per_cu = 0;          // grid size cap in work-groups per CU (0: the kernel's occupancy)
+  bool batch_tasks = true;    // independent groups of one kernel share one launch (task table)
+  int queue = 0;              // device chunk queue: 0 for batched launches only, 1 always, 2 never
   TileShape tile{0, 0, 0};    // forced tile shape (0 = per-group default / tuned)
 };
 
@@ -89,11 +91,20 @@ class Runtime {
   void tune_load(const std::string& path);
   void tune_save(const std::string& path) const;
   size_t tuned_count() const { return tuned_.size(); }
+  std::string kernelSource(const std::string& name) const {
+    auto it = kernel_src_.find(name);
+    return it == kernel_src_.end() ? std::string() : it->second;
+  }
 
  private:
-  struct GroupRun;
+  struct Task;
   std::vector<Node*> cut_list(const std::vector<Node*>& targets);
-  void run_group(const Group& g, const std::vector<Node*>& targets, const std::vector<std::unique_ptr<Group>>& all);
+  void run_level(const std::vector<const Group*>& level, const std::vector<Node*>& targets);
+  bool prepare_task(const Group& g, const std::vector<Node*>& targets, Task& t);
+  void run_batch(const std::vector<Task*>& batch);
+  void fold_results(const std::vector<Task*>& batch, void* parts, void* tasks_dev, void* starts_dev, int maxn);
+  void* device_table(const std::string& bytes);  // immutable device copy of `bytes`, cached by content
+  void* queue();                                  // the compute stream's chunk queue (zeroed once)
   void scalar_task(const Group& g);
   void ensure_input(Node* n);
   std::vector<Node*> d2_outputs(const Group& g, const std::vector<Node*>& targets);
@@ -121,8 +132,12 @@ class Runtime {
   std::string tune_key(const Group& g) const;
   std::map<std::string, TileShape> tuned_;
   std::map<std::string, std::string> kernel_group_;  // kernel name -> tune key
+  std::map<std::string, std::string> kernel_src_;    // kernel name -> generated source
 
   std::unordered_map<Node*, void*> resident_;
+  std::unordered_map<std::string, void*> tables_;
+  size_t table_bytes_ = 0;
+  void* queue_ = nullptr;
   std::recursive_mutex mtx_;
 };
 

@@ -661,6 +661,13 @@ def kernel_times():
         _err(None)
     return _json.loads(s.decode())
 
+def kernel_source(name):
+    """Generated HIP source of a kernel built in this process."""
+    s = _lib.ma_kernelSource(name.encode())
+    if s is None:
+        _err(None)
+    return s.decode()
+
 # ---- ctypes prototypes (python/map.py:723-830) ------------------------------------------
 
 
@@ -707,6 +714,7 @@ _proto('ma_compilePlan', [ct.POINTER(Node), ct.c_int], ct.c_int)
 _proto('ma_stats', [], ct.c_char_p)
 _proto('ma_lastEvalMs', [], ct.c_double)
 _proto('ma_kernelTimes', [], ct.c_char_p)
+_proto('ma_kernelSource', [ct.c_char_p], ct.c_char_p)
 _proto('ma_autotune', [ct.POINTER(Node), ct.c_int, ct.c_int], ct.c_char_p)
 _proto('ma_tuneFile', [ct.c_char_p, ct.c_int], ct.c_int)
 _proto('ma_genDem', [Array, Array, ct.POINTER(ct.c_double), ct.c_int, ct.c_ulonglong], Node)

@@ -56,3 +56,13 @@ def life(M, state, n):
 
 def wsum(M, r1, r2, r3, r4):
     return 0 + 0.1 * r1 + r2 * 0.2 + 0.3 * r3 + r4 * 0.4 + 0  # python/wsum.py:11
+
+
+def batch_tile(M, dem, zones, weights, nz=16, thr=0.5):
+    """Per-tile pipeline of the batched workload: a 5x5 weighted focal sum,
+    the slope of it, a threshold mask, and per-zone statistics of the mask
+    and of the slope."""
+    s = M.convolve(dem, weights)
+    sl = slope(M, s)
+    mask = sl > thr
+    return mask, M.zstats(mask, zones, nz), M.zstats(sl, zones, nz)

@@ -83,6 +83,24 @@ def test_focal_zonal_fusion_is_opt_in(M):
         M.set_option("focal_zonal", 0)
 
 
+def test_focal_zonal_keeps_focal_on_focal_apart(M):
+    """With the focal->zonal extension a focal producer read through a halo
+    (a focal op on a focal result) still stays in its own group."""
+    d = M.from_array(np.zeros((115, 52), np.float32), (64, 64))
+    z = M.from_array(np.zeros((115, 52), np.int32), (64, 64))
+    w = (np.ones((5, 5)) / 64).tolist()
+    for fz, ngroups in [(0, 4), (1, 2)]:
+        M.set_option("focal_zonal", fz)
+        try:
+            p = M.plan(*scripts.batch_tile(M, d, z, w))
+        finally:
+            M.set_option("focal_zonal", 0)
+        assert len(p["groups"]) == ngroups, [g["kinds"] for g in p["groups"]]
+        assert all("error" not in g for g in p["groups"]), [g.get("error") for g in p["groups"]]
+        assert all(g["kinds"].count("Convolution") <= 2 and g["halo"] in ([2, 2], [1, 1], [0, 0])
+                   for g in p["groups"] if "halo" in g)
+
+
 def test_kernels_compile_for_gfx950(M):
     r = dem(M)
     u = M.from_array(np.zeros((64, 64), np.uint8), (32, 32))
@@ -90,6 +108,7 @@ def test_kernels_compile_for_gfx950(M):
     exprs = [scripts.hillshade(M, r), scripts.life(M, u, 2), M.zsum(r), M.zmax(u), M.zmin(r * 2),
              M.zstats(r, zones, 16), M.fmax(r, [[1, 1, 1], [1, 1, 1], [1, 1, 1]]), r(  [1, -1]) + r([-1, 0]),
              M.con(r > 0, r, -r), M.astype(r, M.S32) % 7]
+    exprs += list(scripts.batch_tile(M, r, zones, (np.ones((5, 5)) / 64).tolist()))
     for e in exprs:
         assert M.compile_plan(e) >= 1
 
@@ -99,5 +118,5 @@ def test_tile_fits_the_block(M):
     for side, bs in [(96, 48), (1024, 1024), (64, 16)]:
         u = M.from_array(np.zeros((side, side), np.uint8), (bs, bs))
         src = M.plan(scripts.life(M, u, 1))["groups"][0]["source"]
-        tx, ty, vx = map(int, re.search(r"focal_body<P, (\d+), (\d+), (\d+)>", src).groups())
+        tx, ty, vx = map(int, re.search(r"run_chunks<P, (\d+), (\d+), (\d+), true>", src).groups())
         assert tx <= bs and ty <= bs and 256 % (tx // vx) == 0, (bs, tx, ty, vx)

@@ -160,3 +160,76 @@ def test_zstats(G, O):
     assert np.array_equal(t["max"], ref["max"])
     tol = 1e-12 * np.maximum(1.0, np.abs(ref["sum"])) * 64
     assert np.all(np.abs(t["sum"] - ref["sum"]) <= tol), np.max(np.abs(t["sum"] - ref["sum"]))
+
+
+def _batch_cases(n, seed, bs=64):
+    """Tiles of uneven sizes (not block multiples; edge remainders 0 or >= the
+    5x5 halo), each with its DEM, 16 Voronoi zones and 5x5 weights in 1/64."""
+    from oracle import ref as O
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cases = []
+    while len(cases) < n:
+        w, h = (int(round(np.exp(rng.uniform(np.log(40), np.log(330))))) for _ in range(2))
+        if w % bs == 1 or h % bs == 1:
+            continue
+        g = np.stack([rng.uniform(0, w, 6), rng.uniform(0, h, 6), rng.uniform(5, 60, 6), rng.uniform(0, 300, 6)], 1)
+        dem = O.dem(0, 0, w, h, g, int(rng.integers(1 << 30)))
+        zones = O.voronoi(0, 0, w, h, rng.integers(0, w, 16), rng.integers(0, h, 16))
+        wts = (rng.integers(-64, 65, (5, 5)) / 64.0).tolist()
+        cases.append((dem, zones, wts))
+    return cases
+
+
+def _run_batch(G, cases, bs, together):
+    built = [scripts.batch_tile(G, G.from_array(d, bs), G.from_array(z, bs), w) for d, z, w in cases]
+    if together:
+        G.eval(*[x for b in built for x in b])
+    else:
+        for b in built:
+            G.eval(*b)
+    return [(G.to_array(m), zm.table(), zs.table()) for m, zm, zs in built]
+
+
+def test_batched_tasks_match_alone_plain_and_oracle(G, O):
+    """Uneven tiles evaluated together (one launch per kernel over a device
+    task table and chunk queue), one by one, and in the plain per-block form
+    give identical masks and identical per-zone tables (also the FP64 sums:
+    partial records are per chunk, so results do not depend on the launch);
+    masks and tables agree with the oracle."""
+    bs = (64, 64)
+    cases = _batch_cases(10, 11)
+    print("case batched tasks", [c[0].shape for c in cases], flush=True)
+    runs = {}
+    try:
+        for name, form, together, fz in [("batched", 1, True, 0), ("alone", 1, False, 0), ("plain", 0, True, 0),
+                                         ("fused", 1, True, 1)]:
+            G.set_option("form", form)
+            G.set_option("focal_zonal", fz)
+            runs[name] = _run_batch(G, cases, bs, together)
+            print("  ran", name, G.stats()["launches"], flush=True)
+    finally:
+        G.set_option("form", 1)
+        G.set_option("focal_zonal", 0)
+    base = runs["batched"]
+    for name, r in runs.items():
+        for i, ((m, zm, zs), (m0, zm0, zs0)) in enumerate(zip(r, base)):
+            assert_exact(m, m0, "%s tile %d mask" % (name, i))
+            for k in ("sum", "min", "max", "count"):
+                assert np.array_equal(zm[k], zm0[k]), (name, i, "mask", k)
+                assert np.array_equal(zs[k], zs0[k], equal_nan=True), (name, i, "slope", k)
+    for i, (d, z, w) in enumerate(cases):
+        od = O.from_array(d, bs)
+        sl = scripts.slope(O, O.convolve(od, w))
+        m_ref = (sl.val > np.float32(0.5)).astype(np.uint8)
+        # cells whose slope lies within 4 ulp of the threshold may round either way
+        amb = np.abs(sl.val.astype(np.float64) - 0.5) <= 4 * np.spacing(np.float32(0.5))
+        m, zm, zs = base[i]
+        assert np.array_equal(m[~amb], m_ref[~amb]), "tile %d mask vs oracle" % i
+        ref = O.zstats(O.from_array(m_ref, bs), O.from_array(z, bs), 16)
+        assert np.array_equal(zm["count"], ref["count"])
+        assert np.all(np.abs(zm["sum"] - ref["sum"]) <= np.bincount(z[amb], minlength=16)[:16]), i
+        rs = O.zstats(sl, O.from_array(z, bs), 16)
+        assert np.array_equal(zs["count"], rs["count"])
+        tol = 4 * 2.0 ** -23 * np.maximum(1.0, np.abs(rs["sum"])) + 1e-9
+        # slope values: within 4 ulp of the oracle's, so sums within 4 ulp relative
+        assert np.all(np.abs(zs["sum"] - rs["sum"]) <= tol * 2), i

@@ -173,3 +173,86 @@ def test_border_rule_matches_oracle():
                     gx = (blk % nb[0]) * bs[0] + sx
                     gy = (blk // nb[0]) * bs[1] + sy
                     assert (gx, gy) == O.src_cell(ds, bs, bx, by, x, y)
+
+
+# ---- chunk split, partial slots and task lookup (device/mapx_dev.hiph run_chunks) ----
+
+def make_walk(nwork, G, g):
+    if G % 8 == 0 and G >= 8:
+        x, r, g8 = g & 7, g >> 3, G >> 3
+        c = (nwork + 7) >> 3
+        b, e = x * c, min(x * c + c, nwork)
+        return range(b + r, max(e, b + r), g8)
+    return range(g, nwork, G)
+
+
+def chunk_tiles(L, cl):
+    """Tiles (block, tile) of chunk cl of a launch, and its partial slot (run_chunks)."""
+    b = L["blk0"] + cl // L["CPB"]
+    band = cl % L["CPB"]
+    tiles = L["TB0"] * L["TB1"]
+    t0, t1 = band * L["CH"], min(band * L["CH"] + L["CH"], tiles)
+    return [(b, t) for t in range(t0, t1)], L["part0"] + cl
+
+
+def task_of(starts, c, t):
+    if starts[t + 1] > c:
+        return t
+    lo, hi = t + 1, len(starts) - 2
+    while lo < hi:
+        m = (lo + hi + 1) >> 1
+        if starts[m] <= c:
+            lo = m
+        else:
+            hi = m - 1
+    return lo
+
+
+@pytest.mark.parametrize("nb,tb,ch", [((4, 3), (4, 64), 16), ((1, 1), (1, 1), 1), ((5, 2), (3, 7), 4),
+                                      ((2, 2), (4, 16), 64), ((3, 3), (2, 5), 1)])
+def test_chunks_cover_tiles_once_and_slots_match_forms(nb, tb, ch):
+    tiles = tb[0] * tb[1]
+    ch = min(ch, tiles)
+    cpb = (tiles + ch - 1) // ch
+    nblk = nb[0] * nb[1]
+    L = dict(TB0=tb[0], TB1=tb[1], CH=ch, CPB=cpb, blk0=0, part0=0)
+    batched = {}
+    for G in (8, 16, 24, 1, 3):  # static split: every chunk exactly once for any grid
+        seen = []
+        for g in range(G):
+            seen += list(make_walk(nblk * cpb, G, g))
+        assert sorted(seen) == list(range(nblk * cpb)), G
+    for cl in range(nblk * cpb):
+        ts, slot = chunk_tiles(L, cl)
+        for bt in ts:
+            assert bt not in batched
+            batched[bt] = slot
+    assert len(batched) == nblk * tiles
+    # plain form: one launch per block, chunks land in the same slots
+    for b in range(nblk):
+        Lp = dict(L, blk0=b, part0=b * cpb)
+        for cl in range(cpb):
+            ts, slot = chunk_tiles(Lp, cl)
+            for bt in ts:
+                assert batched[bt] == slot
+
+
+def test_task_lookup_in_batched_launch():
+    rng = __import__("numpy").random.default_rng(0)
+    sizes = [int(s) for s in rng.integers(0, 9, 300)]
+    sizes = [s for s in sizes if s > 0]
+    starts = [0]
+    for s in sizes:
+        starts.append(starts[-1] + s)
+    total = starts[-1]
+    want = [i for i, s in enumerate(sizes) for _ in range(s)]
+    for G in (8, 64, 3):
+        for g in range(G):
+            t = 0
+            for c in make_walk(total, G, g):
+                t = task_of(starts, c, t)
+                assert t == want[c]
+    t = 0
+    for c in sorted(rng.choice(total, 200, replace=False)):  # queue order: increasing per work-group
+        t = task_of(starts, int(c), t)
+        assert t == want[c]
