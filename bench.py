@@ -62,6 +62,281 @@ def cpu_baseline(s0_full, bs, budget_s=12.0):
                       "oracle/oracle.c or_life with OpenMP" % (side, side, gens, bs[0], bs[1])}
 
 
+def _kernel_profile(M, fn):
+    """Runs fn() once with every launch timed; returns {kernel: record}."""
+    M.set_option("profile", 1)
+    M.kernel_times()
+    fn()
+    kt = M.kernel_times()
+    M.set_option("profile", 0)
+    return kt
+
+
+def hillshade_zonal_bench(M, side=32768, block=1024, reps=3):
+    """Hillshade + per-zone statistics over 256 Voronoi zones on a side^2 FP32
+    DEM (seeded Gaussians + fBm) and an S32 zone raster, fused into one kernel
+    (focal_zonal) and in the reference's grouping (hillshade materialised)."""
+    import numpy as np
+    from map_amd import workloads as W
+    rng = np.random.Generator(np.random.PCG64(2))
+    dem = M.gen_dem((side, side), (block, block), W.dem_gaussians(rng, side, side, ref_side=4096.0 * side / 4096), 2)
+    rz = np.random.Generator(np.random.PCG64(3))
+    zones = M.gen_zones((side, side), (block, block), rz.integers(0, side, 256), rz.integers(0, side, 256))
+    cells = side * side
+    out = {}
+    for name, fz in (("fused", 1), ("unfused", 0)):
+        M.set_option("focal_zonal", fz)
+        zs = W.hillshade_zonal(M, dem, zones, 256)
+        M.eval(zs)  # compile, warm
+        walls = []
+        for _ in range(reps):
+            M.release(zs)
+            t0 = time.perf_counter()
+            M.eval(zs)
+            walls.append(time.perf_counter() - t0)
+        kt = _kernel_profile(M, lambda: (M.release(zs), M.eval(zs)))
+        main = max(kt.items(), key=lambda kv: kv[1]["total_ms"])
+        dev_ms = sum(v["total_ms"] for v in kt.values())
+        t = zs.table()
+        out[name] = {"wall_ms": round(min(walls) * 1e3, 3), "device_ms": round(dev_ms, 3),
+                     "gcells_s": round(cells / min(walls) / 1e9, 2),
+                     "kernels": {k: round(v["total_ms"] / v["count"], 4) for k, v in kt.items()},
+                     "main_kernel_ms": round(main[1]["total_ms"] / main[1]["count"], 4),
+                     "count_total": int(t["count"].sum())}
+        del zs
+    M.set_option("focal_zonal", 0)
+    f = out["fused"]
+    alg = 8.0 * cells  # DEM 4 B + zone 4 B per cell
+    f["achieved_gbs"] = round(alg / (f["main_kernel_ms"] * 1e-3) / 1e9, 1)
+    f["hbm_frac"] = round(f["achieved_gbs"] / HBM_PEAK_GBS, 4)
+    return {"workload": "hillshade + per-zone stats (256 Voronoi zones), %dx%d f32 DEM, s32 zones, blocks %dx%d"
+                        % (side, side, block, block), "algorithmic_bytes_per_cell": 8, **out}
+
+
+def batched_tiles_bench(M, ntiles=2048, block=1024, seed=4):
+    """2048 independent tiles of log-uniform sides in [256, 4096]: 5x5
+    weighted focal sum -> slope -> mask slope > 0.5 -> per-zone sums of the mask
+    over 16 Voronoi zones (U8). Batched (one launch per kernel over a device
+    task table) against the plain per-block form, identical outputs."""
+    import numpy as np
+    from map_amd import workloads as W
+    rng = np.random.Generator(np.random.PCG64(seed))
+    tiles, cells = [], 0
+    for i in range(ntiles):
+        w, h = (W.fit_side(int(round(np.exp(rng.uniform(np.log(256), np.log(4096))))), block, 2) for _ in range(2))
+        trng = np.random.Generator(np.random.PCG64([seed, i]))
+        dem = M.gen_dem((w, h), (block, block), W.dem_gaussians(trng, w, h), 1000 + i)
+        zones = M.gen_zones((w, h), (block, block), trng.integers(0, w, 16), trng.integers(0, h, 16), M.U8)
+        wts = (trng.integers(-64, 65, (5, 5)) / 64.0).tolist()
+        tiles.append((dem, zones, wts))
+        cells += w * h
+    M.set_option("focal_zonal", 1)
+    res = {}
+    try:
+        for name, form in (("batched", 1), ("plain", 0)):
+            M.set_option("form", form)
+            outs = [W.batch_tile_sums(M, d, z, w) for d, z, w in tiles]
+            t0 = time.perf_counter()
+            M.eval(*outs)  # plan, compile, first run
+            first = time.perf_counter() - t0
+            walls = []
+            for _ in range(2):
+                for o in outs:
+                    M.release(o)
+                t0 = time.perf_counter()
+                M.eval(*outs)
+                walls.append(time.perf_counter() - t0)
+            st = M.stats()
+            res[name] = {"wall_ms": round(min(walls) * 1e3, 2), "first_eval_ms": round(first * 1e3, 1),
+                         "gcells_s": round(cells / min(walls) / 1e9, 2),
+                         "sums": [o.table()["sum"] for o in outs]}
+            del outs
+    finally:
+        M.set_option("form", 1)
+        M.set_option("focal_zonal", 0)
+    same = all(np.array_equal(a, b) for a, b in zip(res["batched"].pop("sums"), res["plain"].pop("sums")))
+    return {"workload": "%d tiles, sides log-uniform [256, 4096], 5x5 weighted sum -> slope -> mask -> 16-zone sums, "
+                        "blocks %dx%d" % (ntiles, block, block),
+            "cells": cells, "outputs_identical": bool(same),
+            "speedup_vs_plain": round(res["plain"]["wall_ms"] / res["batched"]["wall_ms"], 2), **res}
+
+
+class DemCache:
+    """DEM id -> resident DEM (and zone raster), least recently used evicted past a byte budget."""
+
+    def __init__(self, M, block, budget):
+        import collections
+        self.M, self.block, self.budget = M, block, budget
+        self.d = collections.OrderedDict()
+        self.bytes = 0
+        self.hits = self.misses = 0
+
+    def side(self, k):
+        import numpy as np
+        from map_amd import workloads as W
+        r = np.random.Generator(np.random.PCG64([7, k]))
+        return tuple(W.fit_side(int(round(np.exp(r.uniform(np.log(512), np.log(16384))))), self.block, 3)
+                     for _ in range(2))
+
+    def get(self, k, zones):
+        import numpy as np
+        from map_amd import workloads as W
+        M = self.M
+        e = self.d.get(k)
+        if e is not None and (not zones or e[1] is not None):
+            self.hits += 1
+            self.d.move_to_end(k)
+            return e
+        self.misses += 1
+        w, h = self.side(k)
+        r = np.random.Generator(np.random.PCG64([8, k]))
+        dem = e[0] if e else M.gen_dem((w, h), (self.block, self.block), W.dem_gaussians(r, w, h), 5000 + k)
+        z = M.gen_zones((w, h), (self.block, self.block), r.integers(0, w, 64), r.integers(0, h, 64)) if zones else None
+        nb = (w * h * 4) * (2 if zones else 1)
+        if e:
+            self.bytes -= e[2]
+        self.d[k] = (dem, z, nb, w * h)
+        self.d.move_to_end(k)
+        self.bytes += nb
+        while self.bytes > self.budget and len(self.d) > 1:
+            _, old = self.d.popitem(last=False)
+            self.bytes -= old[2]
+        return self.d[k]
+
+
+SERVE_KINDS = ("hillshade", "slope_aspect", "mean7", "zstats")
+SERVE_MIX = (0.4, 0.3, 0.2, 0.1)
+SERVE_BYTES = {"hillshade": 8, "slope_aspect": 12, "mean7": 8, "zstats": 8}
+
+
+def serve_stream(n, rate, seed, ndem=500):
+    """n requests: Poisson arrival times at `rate`, kinds from the mix, DEM ids Zipf(1.1) over ndem."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = np.cumsum(rng.exponential(1.0 / rate, n)) if rate > 0 else np.zeros(n)
+    kinds = rng.choice(len(SERVE_KINDS), n, p=SERVE_MIX)
+    p = 1.0 / np.arange(1, ndem + 1) ** 1.1
+    ids = rng.choice(ndem, n, p=p / p.sum())
+    return [(float(t[i]), SERVE_KINDS[kinds[i]], int(ids[i])) for i in range(n)]
+
+
+def serve_run(M, cache, stream, max_inflight=64, drain_s=20.0):
+    """Open loop: each request is submitted at its arrival time (or as soon as
+    the host gets to it) and completes when the server retires it. Latency =
+    completion - scheduled arrival."""
+    import numpy as np
+    from map_amd import workloads as W
+    srv = M.Server(hbm_budget=64 << 30, max_inflight=max_inflight)
+    lat, cells, nbytes = [], 0, 0
+    arr = {}
+    i, n = 0, len(stream)
+    t0 = time.monotonic()
+    try:
+        while len(lat) < n:
+            now = time.monotonic() - t0
+            while i < n and stream[i][0] <= now:
+                ta, kind, k = stream[i]
+                dem, z, _, c = cache.get(k, kind == "zstats")
+                rid = srv.submit(*W.serve_request(M, kind, dem, z))
+                arr[rid] = ta
+                cells += c
+                nbytes += c * SERVE_BYTES[kind]
+                i += 1
+            wait = int(min(2000, max(0, (stream[i][0] - (time.monotonic() - t0)) * 1e6))) if i < n else 2000
+            for rid, ts, tadm, td in srv.step(wait):
+                lat.append(td - t0 - arr.pop(rid))
+                srv.result(rid)
+            if i >= n and time.monotonic() - t0 > stream[-1][0] + drain_s:
+                break
+        wall = time.monotonic() - t0
+        st = srv.stats()
+    finally:
+        srv.close()
+    lat = np.array(lat) * 1e3
+    return {"requests": n, "completed": int(len(lat)), "wall_s": round(wall, 3),
+            "req_s": round(len(lat) / wall, 1), "gcells_s": round(cells / wall / 1e9, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3) if len(lat) else None,
+            "p99_ms": round(float(np.percentile(lat, 99)), 3) if len(lat) else None,
+            "mean_tasks_per_step": round(st["group_tasks"] / max(1, st["steps"]), 2),
+            "waits_memory": st["waits_memory"]}
+
+
+def sequential_run(M, cache, stream, form):
+    """The same requests one at a time (ma_eval per request) in the given form."""
+    import numpy as np
+    from map_amd import workloads as W
+    M.set_option("form", form)
+    lat = []
+    cells = 0
+    t0 = time.monotonic()
+    try:
+        for ta, kind, k in stream:
+            dem, z, _, c = cache.get(k, kind == "zstats")
+            outs = W.serve_request(M, kind, dem, z)
+            t1 = time.monotonic()
+            M.eval(*outs)
+            lat.append(time.monotonic() - t1)
+            cells += c
+            del outs
+    finally:
+        M.set_option("form", 1)
+    wall = time.monotonic() - t0
+    lat = np.array(lat) * 1e3
+    return {"requests": len(stream), "req_s": round(len(stream) / wall, 1), "gcells_s": round(cells / wall / 1e9, 2),
+            "service_p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "service_p99_ms": round(float(np.percentile(lat, 99)), 3)}
+
+
+def serve_bench(M, block=1024, seconds=5.0, seed=9):
+    """Served mix (40% hillshade, 30% slope+aspect, 20% 7x7 mean, 10% 64-zone
+    stats; DEM sides log-uniform [512, 16384], 500 DEM ids, Zipf 1.1): the
+    plain (per-block launches) and batched forms one request at a time, then
+    the server at Poisson rates around and above the sequential capacity."""
+    cache = DemCache(M, block, 96 << 30)
+    warm = serve_stream(600, 0, seed + 1)
+    sequential_run(M, cache, warm, 1)  # DEMs of the popular ids resident, kernels compiled
+    probe = serve_stream(200, 0, seed + 2)
+    plain = sequential_run(M, cache, probe, 0)
+    seq = sequential_run(M, cache, probe, 1)
+    out = {"workload": "served mix over 500 DEMs (sides log-uniform [512, 16384], Zipf 1.1), blocks %dx%d"
+                       % (block, block), "sequential_plain": plain, "sequential_batched": seq, "served": []}
+    for mult in (0.5, 1.0, 2.0, 4.0):
+        rate = max(1.0, seq["req_s"] * mult)
+        n = int(min(20000, max(50, rate * seconds)))
+        r = serve_run(M, cache, serve_stream(n, rate, seed + int(mult * 10)))
+        r["offered_req_s"] = round(rate, 1)
+        out["served"].append(r)
+    out["dem_cache"] = {"hits": cache.hits, "misses": cache.misses, "resident_gb": round(cache.bytes / 2**30, 1)}
+    return out
+
+
+def hillshade_bench(M, side=4096, block=512, reps=5):
+    """The plain hillshade script (python/hill.py) on a side^2 FP32 DEM in
+    block^2 blocks: one fused focal+local group, plain and batched forms."""
+    import numpy as np
+    from map_amd import workloads as W
+    rng = np.random.Generator(np.random.PCG64(0))
+    dem = M.gen_dem((side, side), (block, block), W.dem_gaussians(rng, side, side), 0)
+    out = {}
+    for name, form in (("batched", 1), ("plain", 0)):
+        M.set_option("form", form)
+        hs = W.hillshade(M, dem)
+        M.eval(hs)
+        walls = []
+        for _ in range(reps):
+            M.release(hs)
+            t0 = time.perf_counter()
+            M.eval(hs)
+            walls.append(time.perf_counter() - t0)
+        kt = _kernel_profile(M, lambda: (M.release(hs), M.eval(hs)))
+        k = max(kt.values(), key=lambda v: v["total_ms"])
+        out[name] = {"wall_ms": round(min(walls) * 1e3, 3), "gcells_s": round(side * side / min(walls) / 1e9, 2),
+                     "kernel_ms_per_launch": round(k["total_ms"] / k["count"], 4), "launches": k["count"]}
+        del hs
+    M.set_option("form", 1)
+    return {"workload": "hillshade %dx%d f32, blocks %dx%d" % (side, side, block, block), **out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +350,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-gens", type=int, default=8, help="generations timed once in the plain form")
     ap.add_argument("--tune", type=int, default=1, help="autotune the tile shape before timing")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the Life headline (skip hillshade, hillshade+zonal, batched tiles, serving)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a U8 copy of the input raster (a local cast kernel), for counter calibration")
     args = ap.parse_args()
@@ -190,6 +467,19 @@ def main():
                         "plain_launches": args.plain_gens * (args.side // args.block) ** 2}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(s0, bs)
+    if not args.no_extras:
+        M.release(out)
+        M.set_option("graph", 0)
+        extras = {}
+        for name, fn in (("hillshade", hillshade_bench), ("hillshade_zonal", hillshade_zonal_bench),
+                         ("batched_tiles", batched_tiles_bench), ("serve", serve_bench)):
+            t0 = time.perf_counter()
+            try:
+                extras[name] = fn(M)
+            except Exception as e:  # reported, the headline stands
+                extras[name] = {"error": "%s: %s" % (type(e).__name__, e)}
+            print("bench: %s done in %.1fs" % (name, time.perf_counter() - t0), file=sys.stderr, flush=True)
+        res["extras"] = extras
     print(json.dumps(res), flush=True)
 
 

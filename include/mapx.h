@@ -127,7 +127,8 @@ int ma_release(ma_node_t *node);
  * "wg_per_cu", "profile" (time every launch), "batch_tasks" (independent groups
  * of one kernel in one evaluation share one launch through a device task
  * table), "queue" (device chunk queue: 0 batched launches only, 1 always,
- * 2 never). Returns 0 on success. */
+ * 2 never), "chunk_tiles" (tiles a work-group takes at a time in groups
+ * without reductions; 0: 8 with the queue, else 1). Returns 0 on success. */
 int ma_setOption(const char *key, long long value);
 long long ma_getOption(const char *key);
 /* Fusion groups and generated kernel sources of an evaluation, as JSON, without
@@ -147,6 +148,25 @@ double ma_lastEvalMs(void);
 /* Durations of the kernels launched since the last call while the "profile"
  * option is set (HIP events around each launch), as JSON by kernel name. */
 const char *ma_kernelTimes(void);
+/* ---- serving (extension) ----
+ * Requests (sets of target nodes) are evaluated as they arrive: admitted while
+ * their HBM footprint fits hbm_budget bytes (0: 80% of free HBM) and fewer than
+ * max_inflight run; each step enqueues the next level of every admitted
+ * request, batching the fused groups of all requests by kernel, and retires
+ * the requests the GPU has finished (waiting up to wait_us for one). While a
+ * server is open, ma_eval is refused. */
+int ma_serveOpen(long long hbm_budget, int max_inflight);
+/* Queues a request; returns its id (>= 0) or -1. */
+long long ma_serveSubmit(ma_node_t **targets, int num);
+/* One step; returns the number of requests retired in it, or -1. */
+int ma_serveStep(int wait_us);
+/* Copies up to cap retired requests (oldest first): id and steady-clock
+ * (CLOCK_MONOTONIC) seconds of submission, admission and completion. */
+int ma_serveTake(long long *ids, double *t_submit, double *t_admit, double *t_done, int cap);
+/* Server counters as JSON. */
+const char *ma_serveStats(void);
+/* Runs every submitted request to completion and closes the server. */
+int ma_serveClose(void);
 /* Generated HIP source of a kernel this process has built (by kernel name), or NULL. */
 const char *ma_kernelSource(const char *name);
 /* Autotuner: evaluates the nodes once per candidate tile shape (width,

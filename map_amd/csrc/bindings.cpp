@@ -230,6 +230,10 @@ int ma_setOption(const char* key, long long value) {
       if (value < 0 || value > 64) throw std::runtime_error("setOption: wg_per_cu in [0, 64]");
       o.wg_per_cu = (int)value;
     } else if (k == "batch_tasks") o.batch_tasks = value != 0;
+    else if (k == "chunk_tiles") {
+      if (value < 0 || value > 4096) throw std::runtime_error("setOption: chunk_tiles in [0, 4096]");
+      o.chunk_tiles = (int)value;
+    }
     else if (k == "queue") {
       if (value < 0 || value > 2) throw std::runtime_error("setOption: queue in {0 batched only, 1 always, 2 never}");
       o.queue = (int)value;
@@ -252,6 +256,7 @@ long long ma_getOption(const char* key) {
   if (k == "wg_per_cu") return o.wg_per_cu;
   if (k == "batch_tasks") return o.batch_tasks;
   if (k == "queue") return o.queue;
+  if (k == "chunk_tiles") return o.chunk_tiles;
   return -1;
 }
 
@@ -319,6 +324,52 @@ const char* ma_kernelTimes(void) {
     g_json = s.str();
     return g_json.c_str();
   }, (const char*)nullptr);
+}
+
+int ma_serveOpen(long long hbm_budget, int max_inflight) {
+  return guard([&] {
+    if (hbm_budget < 0) throw std::runtime_error("serveOpen: negative budget");
+    R().serveOpen((size_t)hbm_budget, max_inflight);
+    return 0;
+  }, -1);
+}
+
+long long ma_serveSubmit(ma_node_t** vec, int num) {
+  return guard([&] {
+    if (num <= 0 || !vec) throw std::runtime_error("serveSubmit: no nodes");
+    std::vector<Node*> v;
+    for (int i = 0; i < num; i++) v.push_back(need(vec[i]));
+    return R().serveSubmit(v);
+  }, (long long)-1);
+}
+
+int ma_serveStep(int wait_us) {
+  return guard([&] { return R().serveStep(wait_us); }, -1);
+}
+
+int ma_serveTake(long long* ids, double* t_submit, double* t_admit, double* t_done, int cap) {
+  return guard([&] {
+    if (cap < 0) throw std::runtime_error("serveTake: negative capacity");
+    auto d = R().serveTake((size_t)cap);
+    for (size_t i = 0; i < d.size(); i++) {
+      if (ids) ids[i] = d[i].id;
+      if (t_submit) t_submit[i] = d[i].t_submit;
+      if (t_admit) t_admit[i] = d[i].t_admit;
+      if (t_done) t_done[i] = d[i].t_done;
+    }
+    return (int)d.size();
+  }, -1);
+}
+
+const char* ma_serveStats(void) {
+  return guard([&] {
+    g_json = R().serveStats();
+    return g_json.c_str();
+  }, (const char*)nullptr);
+}
+
+int ma_serveClose(void) {
+  return guard([&] { R().serveClose(); return 0; }, -1);
 }
 
 const char* ma_kernelSource(const char* name) {

@@ -343,6 +343,21 @@ void record_flush() {
   check(hipGraphLaunch(exec, s.stream), "hipGraphLaunch");
 }
 
+void* event_record() {
+  hipEvent_t e = nullptr;
+  check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+  check(hipEventRecord(e, S().stream), "hipEventRecord");
+  return (void*)e;
+}
+bool event_done(void* e) {
+  hipError_t r = hipEventQuery((hipEvent_t)e);
+  if (r == hipSuccess) return true;
+  if (r == hipErrorNotReady) return false;
+  check(r, "hipEventQuery");
+  return false;
+}
+void event_destroy(void* e) { if (e) (void)hipEventDestroy((hipEvent_t)e); }
+
 Timer::Timer() {
   check(hipEventCreate((hipEvent_t*)&a), "hipEventCreate");
   check(hipEventCreate((hipEvent_t*)&b), "hipEventCreate");

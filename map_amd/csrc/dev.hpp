@@ -62,6 +62,11 @@ bool recording();
 size_t graph_cache_size();
 void graph_clear();  // destroys the instantiated graphs (call with the stream idle)
 
+// Events on the compute stream (completion tracking without synchronising).
+void* event_record();          // records a new event on the compute stream
+bool event_done(void* e);      // the stream has passed the event
+void event_destroy(void* e);
+
 // Device timing of the compute stream with HIP events.
 struct Timer {
   void* a = nullptr;

@@ -412,6 +412,7 @@ void* Runtime::residentPtr(Node* n) {
 void Runtime::drop(Node* n) {
   auto it = resident_.find(n);
   if (it == resident_.end()) return;
+  if (pins_.count(n)) return;  // an admitted request still reads it
   // a generated raster has no other source: it lives as long as its node
   if (n->host && n->host->dev == it->second && n->ref > 0) return;
   store.release(it->second);
@@ -521,9 +522,14 @@ std::vector<Node*> Runtime::d2_outputs(const Group& g, const std::vector<Node*>&
   return v;
 }
 
+// Tile shapes are tuned per computation: the group's inputs and nodes (not
+// which of its nodes are kept: every generation of an iterated script shares
+// the key with its last one) and the block size.
 std::string Runtime::tune_key(const Group& g) const {
   const Meta m = g.meta();
-  return g.signature() + "#" + std::to_string(m.bs.arr[0]) + "x" + std::to_string(m.bs.arr[1]);
+  std::string sig = g.signature();
+  sig = sig.substr(0, sig.rfind('|'));
+  return sig + "#" + std::to_string(m.bs.arr[0]) + "x" + std::to_string(m.bs.arr[1]);
 }
 
 TileShape Runtime::tile_for(const Group& g) {
@@ -603,21 +609,36 @@ struct Runtime::Task {
   int nchunks = 0;
 
   // chunk split: zonal groups keep CH fixed (it decides their partial records)
-  void chunking(bool queue) {
-    if (ke->spec.zouts.empty()) L.CH = std::min(tiles, queue ? kQueueChunkTiles : 1);
+  void chunking(bool queue, int chunk_tiles) {
+    if (ke->spec.zouts.empty())
+      L.CH = std::max(1, std::min(tiles, chunk_tiles > 0 ? chunk_tiles : (queue ? kQueueChunkTiles : 1)));
     L.CPB = (tiles + L.CH - 1) / L.CH;
     nchunks = L.nblk * L.CPB;
   }
 };
 
-bool Runtime::prepare_task(const Group& g, const std::vector<Node*>& targets, Task& t) {
-  std::vector<Node*> outs = d2_outputs(g, targets);
-  std::vector<Node*> kouts;
+// D2 nodes a group's kernel writes to HBM (not yet resident), and whether it reduces.
+std::vector<Node*> Runtime::kernel_outputs(const Group& g, const std::vector<Node*>& targets, bool& has_zonal) {
+  std::vector<Node*> outs = d2_outputs(g, targets), kouts;
   for (Node* n : outs)
     if (std::find(g.nodes.begin(), g.nodes.end(), n) != g.nodes.end() && !resident_.count(n)) kouts.push_back(n);
-  bool has_zonal = false;
+  has_zonal = false;
   for (Node* n : g.nodes) if (n->kind == Kind::ZonalReduc || n->kind == Kind::ZonalStats) has_zonal = true;
+  return kouts;
+}
 
+bool Runtime::prepare_task(const Group& g, Eval& e, Task& t) {
+  bool has_zonal = false;
+  std::vector<Node*> kouts;
+  KernelEntry* kep = nullptr;
+  auto pre = e.kern.find(&g);
+  if (pre != e.kern.end()) {
+    kouts = pre->second.first;
+    kep = pre->second.second;
+    has_zonal = kep && !kep->spec.zouts.empty();
+  } else {
+    kouts = kernel_outputs(g, e.targets, has_zonal);
+  }
   for (Node* n : g.ins) {
     if (n->meta.nd == D2) ensure_input(n);
     else if (!n->evaluated) fail("D0 input of node " + std::to_string(n->id) + " is not evaluated");
@@ -630,7 +651,7 @@ bool Runtime::prepare_task(const Group& g, const std::vector<Node*>& targets, Ta
       fail("group outputs differ in raster geometry");
     resident_[n] = store.alloc(rasterBytes(n->meta));
   }
-  KernelEntry& ke = kernel_for(cur_plan_, g, kouts);
+  KernelEntry& ke = kep ? *kep : kernel_for(e.plan.get(), g, kouts);
   const KernelSpec& ks = ke.spec;
   if (ks.form == Form::Focal) check_focal_geometry(gm, ks.H0, ks.H1);
   t.g = &g;
@@ -685,9 +706,49 @@ void* Runtime::queue() {
   return queue_;
 }
 
+void* Runtime::pinned_alloc(size_t bytes) {
+  auto it = pinned_free_.lower_bound(bytes);
+  if (it != pinned_free_.end() && it->first <= bytes * 2) {
+    void* p = it->second;
+    pinned_size_[p] = it->first;
+    pinned_free_.erase(it);
+    return p;
+  }
+  void* p = dev::malloc_host(bytes);
+  pinned_size_[p] = bytes;
+  return p;
+}
+
+void Runtime::pinned_release(void* p) {
+  auto it = pinned_size_.find(p);
+  if (it == pinned_size_.end()) return;
+  pinned_free_.emplace(it->second, p);
+}
+
+void Runtime::distribute(Readback& rb) {
+  for (size_t t = 0; t < rb.outs.size(); t++) {
+    const unsigned char* rec = rb.host + t * rb.PB;
+    for (const ZonalOut& z : rb.outs[t]) {
+      Node* node = z.node;
+      if (node->kind == Kind::ZonalReduc) {
+        Variant a = variant_zero(z.acc_dt);
+        std::memcpy(&a.u, rec + z.result_off, sizeof_dt(z.acc_dt));
+        node->value = variant_convert(a, node->meta.dt);
+      } else {
+        node->table.assign(rec + z.result_off, rec + z.result_off + (size_t)z.nzones * 24);
+      }
+      node->evaluated = true;
+    }
+  }
+  pinned_release(rb.host);
+  rb.host = nullptr;
+}
+
 // Folds the partial records of the tasks of one launch (or one task) in index
-// order and reads the zonal results back.
-void Runtime::fold_results(const std::vector<Task*>& batch, void* parts, void* tasks_dev, void* starts_dev, int maxn) {
+// order and reads the zonal results back: at once, or (async != null) as a
+// pending copy whose values are handed out once the stream has passed it.
+void Runtime::fold_results(const std::vector<Task*>& batch, void* parts, void* tasks_dev, void* starts_dev, int maxn,
+                           std::vector<Readback>* async) {
   const KernelEntry& ke = *batch[0]->ke;
   const KernelSpec& ks = ke.spec;
   const int T = (int)batch.size();
@@ -715,25 +776,19 @@ void Runtime::fold_results(const std::vector<Task*>& batch, void* parts, void* t
     dev::launch(fin, 1, 256, a, z, 7, (unsigned)T);
     cnt.launches++;
   }
-  PauseRecording pause;
-  std::vector<unsigned char> host((size_t)T * PB);
-  dev::d2h(host.data(), res, host.size());
-  dev::sync();
-  cnt.bytes_d2h += (long)host.size();
-  for (int t = 0; t < T; t++) {
-    const unsigned char* rec = host.data() + (size_t)t * PB;
-    for (const ZonalOut& z0 : ks.zouts) {
-      // the task's own node of this zonal output: same position in its group's kernel
-      Node* node = batch[t]->ke->spec.zouts[&z0 - ks.zouts.data()].node;
-      if (node->kind == Kind::ZonalReduc) {
-        Variant a = variant_zero(z0.acc_dt);
-        std::memcpy(&a.u, rec + z0.result_off, sizeof_dt(z0.acc_dt));
-        node->value = variant_convert(a, node->meta.dt);
-      } else {
-        node->table.assign(rec + z0.result_off, rec + z0.result_off + (size_t)z0.nzones * 24);
-      }
-      node->evaluated = true;
-    }
+  Readback rb;
+  rb.PB = PB;
+  for (Task* t : batch) rb.outs.push_back(t->ke->spec.zouts);  // each task's own nodes
+  rb.host = (unsigned char*)pinned_alloc((size_t)T * PB);
+  cnt.bytes_d2h += (long)((size_t)T * PB);
+  if (async) {
+    dev::d2h(rb.host, res, (size_t)T * PB);
+    async->push_back(std::move(rb));
+  } else {
+    PauseRecording pause;
+    dev::d2h(rb.host, res, (size_t)T * PB);
+    dev::sync();
+    distribute(rb);
   }
   if (mid) store.release(mid);
   store.release(res);
@@ -741,7 +796,7 @@ void Runtime::fold_results(const std::vector<Task*>& batch, void* parts, void* t
 
 // Launches a batch of tasks of one kernel: per block (plain form), one task by
 // value, or many tasks from a device task table with a device chunk queue.
-void Runtime::run_batch(const std::vector<Task*>& batch) {
+void Runtime::run_batch(const std::vector<Task*>& batch, std::vector<Readback>* async) {
   HostTimer ht_launch(cnt.ms_launch);
   const KernelEntry& ke = *batch[0]->ke;
   const KernelSpec& ks = ke.spec;
@@ -765,7 +820,7 @@ void Runtime::run_batch(const std::vector<Task*>& batch) {
 
   if (opt.form == FORM_PLAIN) {
     for (Task* t : batch) {
-      t->chunking(false);
+      t->chunking(false, opt.chunk_tiles);
       void* parts = zonal ? store.alloc((size_t)t->nchunks * ks.part_bytes) : nullptr;
       std::vector<std::pair<uint32_t, int>> order;
       for (int b = 0; b < t->L.nblk; b++) order.push_back({morton2(b % t->L.NB0, b / t->L.NB0), b});
@@ -778,7 +833,7 @@ void Runtime::run_batch(const std::vector<Task*>& batch) {
         launch(t->args, L, nul, nul, 1, L.CPB, nul, parts, grid_for(L.CPB));
       }
       if (zonal) {
-        fold_results({t}, parts, nullptr, nullptr, t->nchunks);
+        fold_results({t}, parts, nullptr, nullptr, t->nchunks, async);
         store.release(parts);
       }
     }
@@ -787,11 +842,11 @@ void Runtime::run_batch(const std::vector<Task*>& batch) {
   if (batch.size() == 1) {
     Task* t = batch[0];
     const bool q = opt.queue == 1;
-    t->chunking(q);
+    t->chunking(q, opt.chunk_tiles);
     void* parts = zonal ? store.alloc((size_t)t->nchunks * ks.part_bytes) : nullptr;
     launch(t->args, t->L, nul, nul, 1, t->nchunks, q ? queue() : nul, parts, grid_for(t->nchunks));
     if (zonal) {
-      fold_results(batch, parts, nullptr, nullptr, t->nchunks);
+      fold_results(batch, parts, nullptr, nullptr, t->nchunks, async);
       store.release(parts);
     }
     return;
@@ -802,7 +857,7 @@ void Runtime::run_batch(const std::vector<Task*>& batch) {
   std::vector<int> starts(T + 1, 0);
   int maxn = 0;
   for (int i = 0; i < T; i++) {
-    batch[i]->chunking(q);
+    batch[i]->chunking(q, opt.chunk_tiles);
     starts[i + 1] = starts[i] + batch[i]->nchunks;
     maxn = std::max(maxn, batch[i]->nchunks);
   }
@@ -824,20 +879,27 @@ void Runtime::run_batch(const std::vector<Task*>& batch) {
   std::vector<char> a0(ks.args_size, 0);
   launch(a0, L0, tasks_dev, starts_dev, T, total, q ? queue() : nul, parts, grid_for(total));
   if (zonal) {
-    fold_results(batch, parts, tasks_dev, starts_dev, maxn);
+    fold_results(batch, parts, tasks_dev, starts_dev, maxn, async);
     store.release(parts);
   }
 }
 
-void Runtime::run_level(const std::vector<const Group*>& level, const std::vector<Node*>& targets) {
-  std::vector<Task> tasks(level.size());
+// Runs one level of each evaluation: host scalar groups at once, kernels of
+// all evaluations batched by kernel.
+void Runtime::run_levels(const std::vector<std::pair<Eval*, size_t>>& work, std::vector<Readback>* async) {
+  size_t n = 0;
+  for (auto& w : work) n += w.first->levels[w.second].size();
+  std::vector<Task> tasks(n);
   std::vector<Task*> order;
-  for (size_t i = 0; i < level.size(); i++) {
-    const Group& g = *level[i];
-    cnt.groups++;
-    if (g.numdim() == D0) { scalar_task(g); continue; }
-    if (prepare_task(g, targets, tasks[i])) order.push_back(&tasks[i]);
-  }
+  size_t i = 0;
+  for (auto& w : work)
+    for (const Group* gp : w.first->levels[w.second]) {
+      const Group& g = *gp;
+      cnt.groups++;
+      Task& t = tasks[i++];
+      if (g.numdim() == D0) { scalar_task(g); continue; }
+      if (prepare_task(g, *w.first, t)) order.push_back(&t);
+    }
   // tasks of one kernel form one batch (in order of first appearance)
   std::vector<std::vector<Task*>> batches;
   std::unordered_map<std::string, size_t> at;
@@ -851,10 +913,11 @@ void Runtime::run_level(const std::vector<const Group*>& level, const std::vecto
       batches[it->second].push_back(t);
     }
   }
-  for (auto& b : batches) run_batch(b);
-  for (const Group* g : level)
-    for (Node* n : g->outs)
-      if (n->kind == Kind::Write) write_file(n);
+  for (auto& b : batches) run_batch(b, async);
+  for (auto& w : work)
+    for (const Group* g : w.first->levels[w.second])
+      for (Node* n : g->outs)
+        if (n->kind == Kind::Write) write_file(n);
 }
 
 std::string Runtime::plan_key(const std::vector<Node*>& list, const std::vector<Node*>& targets) const {
@@ -885,50 +948,35 @@ Runtime::KernelEntry& Runtime::kernel_for(Plan* plan, const Group& g, const std:
   return plan->kernels.emplace(key, std::move(e)).first->second;
 }
 
-void Runtime::evaluate(const std::vector<Node*>& targets) {
-  std::lock_guard<std::recursive_mutex> lk(mtx_);
-  if (!dev::initialized()) fail("evaluate: devices not set up (call setupDevices first)");
-  if (targets.empty()) fail("evaluate: nothing to evaluate");
-  release_unreferenced();
-  dag.collect([this](Node* n) { forget(n); });
-  cnt.evals++;
-  dev::Timer t;
-  t.start();
-
-  std::vector<Node*> list = cut_list(targets);
-  struct Uncut {
-    std::vector<Node*>& l;
-    ~Uncut() { for (Node* n : l) n->cut = false; }
-  } uncut{list};
-
+// Plans an evaluation: the sub-DAG cut at resident rasters, its fusion
+// groups (cached by evaluation list), their levels and raster uses. Leaves
+// the cut marks set (the caller clears them with uncut()).
+void Runtime::eval_plan(Eval& e) {
+  e.list = cut_list(e.targets);
   FuseOptions fo;
   fo.code_fusion = opt.code_fusion;
   fo.focal_zonal = opt.focal_zonal;
-  const std::string pkey = plan_key(list, targets);
+  const std::string pkey = plan_key(e.list, e.targets);
   auto pit = plans_.find(pkey);
   if (pit == plans_.end()) {
-    if (plans_.size() >= 64) plans_.clear();  // bounded: plans of old scripts are dropped
-    auto pl = std::make_unique<Plan>();
+    if (plans_.size() >= 64) plans_.clear();  // bounded: plans of old scripts are dropped (live evaluations keep theirs)
+    auto pl = std::make_shared<Plan>();
     {
       HostTimer ht(cnt.ms_fuse);
-      pl->groups = Fuser(fo).fuse(list);
+      pl->groups = Fuser(fo).fuse(e.list);
     }
     pit = plans_.emplace(pkey, std::move(pl)).first;
   }
-  cur_plan_ = pit->second.get();
-  const auto& groups = cur_plan_->groups;
-
+  e.plan = pit->second;
+  const auto& groups = e.plan->groups;
   // consumers of each D2 raster in this evaluation, for early release
-  std::map<Node*, int> uses;
   for (auto& g : groups)
-    for (Node* n : g->ins) if (n->meta.nd == D2) uses[n]++;
-  std::set<Node*> tset(targets.begin(), targets.end());
-  for (Node* t : targets) if (t->kind == Kind::Write) tset.insert(t->prev[0]);
-
+    for (Node* n : g->ins) if (n->meta.nd == D2) e.uses[n]++;
+  e.tset.insert(e.targets.begin(), e.targets.end());
+  for (Node* t : e.targets) if (t->kind == Kind::Write) e.tset.insert(t->prev[0]);
   // levels: a group runs one level after the last group it depends on; the
   // groups of one level are independent, and those sharing a kernel are batched
   std::unordered_map<const Group*, int> lvl;
-  std::vector<std::vector<const Group*>> levels;
   for (auto& g : groups) {
     int l = 0;
     for (Group* p : g->prev) {
@@ -936,29 +984,62 @@ void Runtime::evaluate(const std::vector<Node*>& targets) {
       if (it != lvl.end()) l = std::max(l, it->second + 1);
     }
     lvl[g.get()] = l;
-    if ((int)levels.size() <= l) levels.resize(l + 1);
-    levels[l].push_back(g.get());
+    if ((int)e.levels.size() <= l) e.levels.resize(l + 1);
+    e.levels[l].push_back(g.get());
   }
+}
+
+void Runtime::eval_uncut(Eval& e) {
+  for (Node* n : e.list) n->cut = false;
+}
+
+// Inputs of a level's groups that nothing later in the evaluation reads go back to the store.
+void Runtime::eval_release(Eval& e, size_t level) {
+  for (const Group* g : e.levels[level])
+    for (Node* n : g->ins) {
+      if (n->meta.nd != D2) continue;
+      if (--e.uses[n] == 0 && ext_refs(n) <= 0 && !e.tset.count(n) && n->kind != Kind::Read) drop(n);
+    }
+}
+
+// Intermediates nobody holds and nothing consumes any more.
+void Runtime::eval_finish(Eval& e) {
+  for (auto& g : e.plan->groups)
+    for (Node* n : g->outs)
+      if (n->meta.nd == D2 && n->kind != Kind::Write && ext_refs(n) <= 0 && !e.tset.count(n) && e.uses[n] == 0) drop(n);
+}
+
+void Runtime::evaluate(const std::vector<Node*>& targets) {
+  std::lock_guard<std::recursive_mutex> lk(mtx_);
+  if (!dev::initialized()) fail("evaluate: devices not set up (call setupDevices first)");
+  if (targets.empty()) fail("evaluate: nothing to evaluate");
+  if (serving()) fail("evaluate: a server is open (submit requests to it, or close it first)");
+  release_unreferenced();
+  dag.collect([this](Node* n) { forget(n); });
+  cnt.evals++;
+  dev::Timer t;
+  t.start();
+
+  Eval e;
+  e.targets = targets;
+  struct Uncut {
+    Runtime* r; Eval& e;
+    ~Uncut() { r->eval_uncut(e); }
+  } uncut{this, e};
+  eval_plan(e);
 
   if (opt.graph) dev::record_begin();
   try {
-    for (auto& level : levels) {
-      run_level(level, targets);
-      for (const Group* g : level)
-        for (Node* n : g->ins) {
-          if (n->meta.nd != D2) continue;
-          if (--uses[n] == 0 && ext_refs(n) <= 0 && !tset.count(n) && n->kind != Kind::Read) drop(n);
-        }
+    for (size_t l = 0; l < e.levels.size(); l++) {
+      run_levels({{&e, l}}, nullptr);
+      eval_release(e, l);
     }
     if (dev::recording()) dev::record_flush();
   } catch (...) {
     if (dev::recording()) { try { dev::record_flush(); } catch (...) {} }
     throw;
   }
-  // intermediates nobody holds and nothing consumes any more
-  for (auto& g : groups)
-    for (Node* n : g->outs)
-      if (n->meta.nd == D2 && n->kind != Kind::Write && ext_refs(n) <= 0 && !tset.count(n) && uses[n] == 0) drop(n);
+  eval_finish(e);
   t.stop();
   cnt.ms_last_eval = t.ms();
 }
@@ -1090,7 +1171,6 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   for (Node* t : targets) drop(t);
   // cached plans hold kernels generated with the previous tile shapes
   plans_.clear();
-  cur_plan_ = nullptr;
   if (const char* p = std::getenv("MAPX_TUNE_CACHE")) tune_save(p);
   return rep.str();
 }

@@ -12,6 +12,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -31,6 +32,7 @@ struct Options {
   int wg_per_cu = 0;          // grid size cap in work-groups per CU (0: the kernel's occupancy)
   bool batch_tasks = true;    // independent groups of one kernel share one launch (task table)
   int queue = 0;              // device chunk queue: 0 for batched launches only, 1 always, 2 never
+  int chunk_tiles = 0;        // tiles per chunk of non-zonal groups (0: 8 with the queue, else 1)
   TileShape tile{0, 0, 0};    // forced tile shape (0 = per-group default / tuned)
 };
 
@@ -61,6 +63,8 @@ class Store {
 class Runtime {
  public:
   static Runtime& get();
+  Runtime();
+  ~Runtime();
 
   Dag dag;
   Options opt;
@@ -91,6 +95,25 @@ class Runtime {
   void tune_load(const std::string& path);
   void tune_save(const std::string& path) const;
   size_t tuned_count() const { return tuned_.size(); }
+
+  // ---- serving: requests evaluated as they arrive, batched across requests ----
+  // A request is a set of target nodes (a script over resident rasters).
+  // Requests are admitted while their HBM footprint fits the budget and fewer
+  // than max_inflight run; every step enqueues the next level of every
+  // admitted request, batching the groups of all requests by kernel, and
+  // retires requests whose work the stream has passed. No step waits for the
+  // GPU except to detect completions (up to wait_us).
+  struct ServeDone {
+    long long id;
+    double t_submit, t_admit, t_done;  // steady-clock seconds (CLOCK_MONOTONIC)
+  };
+  void serveOpen(size_t hbm_budget, int max_inflight);
+  long long serveSubmit(const std::vector<Node*>& targets);
+  int serveStep(int wait_us);
+  std::vector<ServeDone> serveTake(size_t cap);  // up to cap retired requests, oldest first
+  std::string serveStats();
+  void serveClose();
+  bool serving() const { return srv_ != nullptr; }
   std::string kernelSource(const std::string& name) const {
     auto it = kernel_src_.find(name);
     return it == kernel_src_.end() ? std::string() : it->second;
@@ -98,11 +121,38 @@ class Runtime {
 
  private:
   struct Task;
+  // Zonal results of one batch on their way to the host.
+  struct Readback {
+    unsigned char* host = nullptr;  // pinned, one record of PB bytes per task
+    size_t PB = 0;
+    std::vector<std::vector<ZonalOut>> outs;  // per task: its zonal output nodes and record offsets
+  };
+  struct Plan;
+  struct KernelEntry;
+  // One evaluation in progress: its plan, levels and the use counts of its rasters.
+  struct Eval {
+    std::vector<Node*> targets, list;
+    std::shared_ptr<Plan> plan;
+    std::vector<std::vector<const Group*>> levels;
+    std::map<Node*, int> uses;
+    std::set<Node*> tset;
+    // kernels fixed at planning time (requests of the server run later)
+    std::unordered_map<const Group*, std::pair<std::vector<Node*>, KernelEntry*>> kern;
+  };
   std::vector<Node*> cut_list(const std::vector<Node*>& targets);
-  void run_level(const std::vector<const Group*>& level, const std::vector<Node*>& targets);
-  bool prepare_task(const Group& g, const std::vector<Node*>& targets, Task& t);
-  void run_batch(const std::vector<Task*>& batch);
-  void fold_results(const std::vector<Task*>& batch, void* parts, void* tasks_dev, void* starts_dev, int maxn);
+  void eval_plan(Eval& e);
+  void eval_uncut(Eval& e);
+  void eval_release(Eval& e, size_t level);
+  void eval_finish(Eval& e);
+  std::vector<Node*> kernel_outputs(const Group& g, const std::vector<Node*>& targets, bool& has_zonal);
+  void run_levels(const std::vector<std::pair<Eval*, size_t>>& work, std::vector<Readback>* async);
+  bool prepare_task(const Group& g, Eval& e, Task& t);
+  void run_batch(const std::vector<Task*>& batch, std::vector<Readback>* async);
+  void fold_results(const std::vector<Task*>& batch, void* parts, void* tasks_dev, void* starts_dev, int maxn,
+                    std::vector<Readback>* async);
+  void distribute(Readback& rb);
+  void* pinned_alloc(size_t bytes);
+  void pinned_release(void* p);
   void* device_table(const std::string& bytes);  // immutable device copy of `bytes`, cached by content
   void* queue();                                  // the compute stream's chunk queue (zeroed once)
   void scalar_task(const Group& g);
@@ -127,14 +177,20 @@ class Runtime {
   };
   std::string plan_key(const std::vector<Node*>& list, const std::vector<Node*>& targets) const;
   KernelEntry& kernel_for(Plan* plan, const Group& g, const std::vector<Node*>& kouts);
-  std::unordered_map<std::string, std::unique_ptr<Plan>> plans_;
-  Plan* cur_plan_ = nullptr;
+  std::unordered_map<std::string, std::shared_ptr<Plan>> plans_;
+  std::multimap<size_t, void*> pinned_free_;
+  std::unordered_map<void*, size_t> pinned_size_;
   std::string tune_key(const Group& g) const;
   std::map<std::string, TileShape> tuned_;
   std::map<std::string, std::string> kernel_group_;  // kernel name -> tune key
   std::map<std::string, std::string> kernel_src_;    // kernel name -> generated source
 
   std::unordered_map<Node*, void*> resident_;
+  std::unordered_map<Node*, int> pins_;  // rasters read by admitted requests: not dropped
+  struct ServeState;
+  struct Request;
+  std::unique_ptr<ServeState> srv_;
+  void serve_retire();
   std::unordered_map<std::string, void*> tables_;
   size_t table_bytes_ = 0;
   void* queue_ = nullptr;

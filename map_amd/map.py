@@ -661,6 +661,55 @@ def kernel_times():
         _err(None)
     return _json.loads(s.decode())
 
+class Server:
+    """Serving loop over the backend: requests (tuples of result rasters /
+    zonal nodes) are submitted as they arrive and evaluated with continuous
+    batching; step() advances the loop and returns the retired requests as
+    (id, t_submit, t_admit, t_done) with CLOCK_MONOTONIC seconds
+    (time.monotonic())."""
+
+    def __init__(self, hbm_budget=0, max_inflight=64):
+        if _lib.ma_serveOpen(int(hbm_budget), int(max_inflight)) != 0:
+            _err(None)
+        self._held = {}
+
+    def submit(self, *targets):
+        ptr, n, keep = _nodes(targets)
+        rid = _lib.ma_serveSubmit(ptr, n)
+        if rid < 0:
+            _err(None)
+        self._held[rid] = targets
+        return rid
+
+    def step(self, wait_us=0):
+        if _lib.ma_serveStep(int(wait_us)) < 0:
+            _err(None)
+        return self.take()
+
+    def take(self, cap=4096):
+        ids = (ct.c_longlong * cap)()
+        ts, ta, td = (ct.c_double * cap)(), (ct.c_double * cap)(), (ct.c_double * cap)()
+        k = _lib.ma_serveTake(ids, ts, ta, td, cap)
+        if k < 0:
+            _err(None)
+        out = [(ids[i], ts[i], ta[i], td[i]) for i in range(k)]
+        return out
+
+    def result(self, rid):
+        """The targets of a retired request (the caller drops them when done)."""
+        return self._held.pop(rid)
+
+    def stats(self):
+        s = _lib.ma_serveStats()
+        if s is None:
+            _err(None)
+        return _json.loads(s.decode())
+
+    def close(self):
+        if _lib.ma_serveClose() != 0:
+            _err(None)
+
+
 def kernel_source(name):
     """Generated HIP source of a kernel built in this process."""
     s = _lib.ma_kernelSource(name.encode())
@@ -715,6 +764,13 @@ _proto('ma_stats', [], ct.c_char_p)
 _proto('ma_lastEvalMs', [], ct.c_double)
 _proto('ma_kernelTimes', [], ct.c_char_p)
 _proto('ma_kernelSource', [ct.c_char_p], ct.c_char_p)
+_proto('ma_serveOpen', [ct.c_longlong, ct.c_int], ct.c_int)
+_proto('ma_serveSubmit', [ct.POINTER(Node), ct.c_int], ct.c_longlong)
+_proto('ma_serveStep', [ct.c_int], ct.c_int)
+_proto('ma_serveTake', [ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_double), ct.POINTER(ct.c_double),
+                        ct.POINTER(ct.c_double), ct.c_int], ct.c_int)
+_proto('ma_serveStats', [], ct.c_char_p)
+_proto('ma_serveClose', [], ct.c_int)
 _proto('ma_autotune', [ct.POINTER(Node), ct.c_int, ct.c_int], ct.c_char_p)
 _proto('ma_tuneFile', [ct.c_char_p, ct.c_int], ct.c_int)
 _proto('ma_genDem', [Array, Array, ct.POINTER(ct.c_double), ct.c_int, ct.c_ulonglong], Node)

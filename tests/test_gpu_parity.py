@@ -233,3 +233,64 @@ def test_batched_tasks_match_alone_plain_and_oracle(G, O):
         tol = 4 * 2.0 ** -23 * np.maximum(1.0, np.abs(rs["sum"])) + 1e-9
         # slope values: within 4 ulp of the oracle's, so sums within 4 ulp relative
         assert np.all(np.abs(zs["sum"] - rs["sum"]) <= tol * 2), i
+
+
+def _serve_results(G, outs):
+    res = []
+    for o in outs:
+        res.append(o.table() if isinstance(o, G.ZoneStats) else G.to_array(o))
+    return res
+
+
+def test_serve_results_same_alone_and_in_mix(G, O):
+    """Requests of the served mix (hillshade, slope+aspect, 7x7 mean, per-zone
+    statistics) over DEMs of different sizes, submitted in waves while earlier
+    ones run (continuous batching, a small HBM budget so that admission waits
+    for memory, a repeated request that shares nodes with one in flight): every
+    result equals the result of the same request evaluated alone, bit for bit."""
+    bs = (256, 256)
+    sizes = [(300, 200), (517, 389), (1024, 640), (96, 700), (777, 515)]
+    rng = np.random.Generator(np.random.PCG64(21))
+    dem_np = [dem_array(w, h, 30 + i) for i, (w, h) in enumerate(sizes)]
+    zone_np = [O.voronoi(0, 0, w, h, rng.integers(0, w, 64), rng.integers(0, h, 64)) for (w, h) in sizes]
+    kinds = ["hillshade", "slope_aspect", "mean7", "zstats"]
+    reqs = [(kinds[int(rng.choice(4, p=[0.4, 0.3, 0.2, 0.1]))], int(rng.integers(len(sizes)))) for _ in range(30)]
+    reqs += [reqs[0], reqs[0]]  # identical requests: shared nodes
+    print("case serve", len(reqs), "requests", flush=True)
+    dems = [G.from_array(d, bs) for d in dem_np]
+    zones = [G.from_array(z, bs) for z in zone_np]
+    srv = G.Server(hbm_budget=6 << 20, max_inflight=6)
+    got = {}
+    try:
+        ids = []
+        for wave in range(0, len(reqs), 5):
+            for kind, di in reqs[wave:wave + 5]:
+                ids.append(srv.submit(*scripts.serve_request(G, kind, dems[di], zones[di])))
+            for rid, *_ in srv.step(0):
+                got[rid] = _serve_results(G, srv.result(rid))
+        while len(got) < len(ids):
+            for rid, *_ in srv.step(2000):
+                got[rid] = _serve_results(G, srv.result(rid))
+        st = srv.stats()
+        print("  serve stats", st, flush=True)
+    finally:
+        srv.close()
+    assert st["completed"] == len(reqs) and st["waits_memory"] > 0
+    for i, (kind, di) in enumerate(reqs):
+        d = G.from_array(dem_np[di], bs)
+        z = G.from_array(zone_np[di], bs)
+        outs = scripts.serve_request(G, kind, d, z)
+        G.eval(*outs)
+        alone = _serve_results(G, outs)
+        for a, b in zip(got[ids[i]], alone):
+            if isinstance(a, dict):
+                for k in ("sum", "min", "max", "count"):
+                    assert np.array_equal(a[k], b[k], equal_nan=True), (i, kind, k)
+            else:
+                assert_exact(a, b, "request %d (%s)" % (i, kind))
+    # zonal counts against the oracle
+    for i, (kind, di) in enumerate(reqs):
+        if kind == "zstats":
+            ref = O.zstats(O.from_array(dem_np[di], bs), O.from_array(zone_np[di], bs), 64)
+            assert np.array_equal(got[ids[i]][0]["count"], ref["count"])
+            assert np.array_equal(got[ids[i]][0]["min"], ref["min"])
