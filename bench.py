@@ -62,6 +62,11 @@ def cpu_baseline(s0_full, bs, budget_s=12.0):
                       "oracle/oracle.c or_life with OpenMP" % (side, side, gens, bs[0], bs[1])}
 
 
+def _progress(msg):
+    """A line on stderr per stage of the long sections."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def _kernel_profile(M, fn):
     """Runs fn() once with every launch timed; returns {kernel: record}."""
     M.set_option("profile", 1)
@@ -130,30 +135,46 @@ def batched_tiles_bench(M, ntiles=2048, block=1024, seed=4):
         wts = (trng.integers(-64, 65, (5, 5)) / 64.0).tolist()
         tiles.append((dem, zones, wts))
         cells += w * h
+        if i % 256 == 255:
+            _progress("batched tiles: %d tiles generated" % (i + 1))
     M.set_option("focal_zonal", 1)
     res = {}
     try:
+        # tile shapes tuned on the first 64 tiles (keys: computation and block size)
+        sample = [W.batch_tile_sums(M, d, z, w) for d, z, w in tiles[:64]]
+        _progress("batched tiles: autotuning on 64 tiles")
+        tuned = M.autotune(*sample, reps=1)["chosen"]
+        del sample
         for name, form in (("batched", 1), ("plain", 0)):
             M.set_option("form", form)
+            M.set_option("graph", 1 if form == 1 else 0)  # steady state replayed from hipGraphs
             outs = [W.batch_tile_sums(M, d, z, w) for d, z, w in tiles]
+            _progress("batched tiles: %s form, first evaluation" % name)
             t0 = time.perf_counter()
             M.eval(*outs)  # plan, compile, first run
             first = time.perf_counter() - t0
-            walls = []
+            walls, devs = [], []
             for _ in range(2):
                 for o in outs:
                     M.release(o)
+                h0 = M.stats()
                 t0 = time.perf_counter()
                 M.eval(*outs)
                 walls.append(time.perf_counter() - t0)
-            st = M.stats()
-            res[name] = {"wall_ms": round(min(walls) * 1e3, 2), "first_eval_ms": round(first * 1e3, 1),
+                devs.append(M.last_eval_ms())
+            h1 = M.stats()
+            res[name] = {"wall_ms": round(min(walls) * 1e3, 2), "device_ms": round(min(devs), 2),
+                         "first_eval_ms": round(first * 1e3, 1),
                          "gcells_s": round(cells / min(walls) / 1e9, 2),
+                         "host_ms_last": {k: round(h1[k] - h0[k], 2) for k in ("ms_plan", "ms_fuse", "ms_codegen", "ms_prepare", "ms_launch")},
+                         "launches_last": h1["launches"] - h0["launches"],
                          "sums": [o.table()["sum"] for o in outs]}
             del outs
     finally:
         M.set_option("form", 1)
+        M.set_option("graph", 0)
         M.set_option("focal_zonal", 0)
+    res["tuned_tiles"] = tuned
     same = all(np.array_equal(a, b) for a, b in zip(res["batched"].pop("sums"), res["plain"].pop("sums")))
     return {"workload": "%d tiles, sides log-uniform [256, 4096], 5x5 weighted sum -> slope -> mask -> 16-zone sums, "
                         "blocks %dx%d" % (ntiles, block, block),
@@ -294,6 +315,7 @@ def serve_bench(M, block=1024, seconds=5.0, seed=9):
     the server at Poisson rates around and above the sequential capacity."""
     cache = DemCache(M, block, 96 << 30)
     warm = serve_stream(600, 0, seed + 1)
+    _progress("serve: warming the DEM cache")
     sequential_run(M, cache, warm, 1)  # DEMs of the popular ids resident, kernels compiled
     probe = serve_stream(200, 0, seed + 2)
     plain = sequential_run(M, cache, probe, 0)
@@ -302,6 +324,7 @@ def serve_bench(M, block=1024, seconds=5.0, seed=9):
                        % (block, block), "sequential_plain": plain, "sequential_batched": seq, "served": []}
     for mult in (0.5, 1.0, 2.0, 4.0):
         rate = max(1.0, seq["req_s"] * mult)
+        _progress("serve: %.0f req/s offered" % rate)
         n = int(min(20000, max(50, rate * seconds)))
         r = serve_run(M, cache, serve_stream(n, rate, seed + int(mult * 10)))
         r["offered_req_s"] = round(rate, 1)
@@ -352,6 +375,8 @@ def main():
     ap.add_argument("--tune", type=int, default=1, help="autotune the tile shape before timing")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the Life headline (skip hillshade, hillshade+zonal, batched tiles, serving)")
+    ap.add_argument("--only", choices=["hillshade", "hillshade_zonal", "batched_tiles", "serve"],
+                    help="run one extra section alone (no Life headline), print its JSON")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a U8 copy of the input raster (a local cast kernel), for counter calibration")
     args = ap.parse_args()
@@ -361,6 +386,11 @@ def main():
 
     from map_amd import map as M
     M.setupDevices("", M.DEV_GPU, "")
+    if args.only:
+        fn = {"hillshade": hillshade_bench, "hillshade_zonal": hillshade_zonal_bench,
+              "batched_tiles": batched_tiles_bench, "serve": serve_bench}[args.only]
+        print(json.dumps({args.only: fn(M)}), flush=True)
+        return
     bs = (args.block, args.block)
     s0 = life_input(args.side, 1)
     S = M.from_array(s0, bs)
@@ -416,6 +446,16 @@ def main():
     M.set_option("profile", 0)
     M.set_option("graph", args.graph)
     kname, krec = max(kt.items(), key=lambda kv: kv[1]["total_ms"])
+    # HBM bytes per launch of this kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # passes scaled by a copy kernel of known size (profiles/life_hbm_traffic.json)
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "life_hbm_traffic.json")) as f:
+            tr = json.load(f)
+        if tr.get("kernel") == kname:
+            traffic = tr["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        pass
     launches = krec["count"]
     kern_ms = krec["total_ms"] / krec["count"]
     alg_bytes = 2.0 * cells  # one U8 read + one U8 write per cell and generation
@@ -438,7 +478,7 @@ def main():
                                % (args.side, args.side, args.gens, bs[0], bs[1]),
                    "form": args.form, "graph": bool(args.graph)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": kname, "kernel_avg_ms": round(kern_ms, 5), "kernel_min_ms": round(krec["min_ms"], 5),
                      "launches": launches, "algorithmic_bytes_per_launch": int(alg_bytes)},
         "device_ms_per_step": round(min(dev_ms), 3),
@@ -477,6 +517,8 @@ def main():
             try:
                 extras[name] = fn(M)
             except Exception as e:  # reported, the headline stands
+                import traceback
+                traceback.print_exc(file=sys.stderr)
                 extras[name] = {"error": "%s: %s" % (type(e).__name__, e)}
             print("bench: %s done in %.1fs" % (name, time.perf_counter() - t0), file=sys.stderr, flush=True)
         res["extras"] = extras

@@ -226,6 +226,7 @@ int ma_setOption(const char* key, long long value) {
     else if (k == "tile_tx") o.tile.TX = (int)value;
     else if (k == "tile_ty") o.tile.TY = (int)value;
     else if (k == "tile_vx") o.tile.VX = (int)value;
+    else if (k == "tile_pf") o.tile.PF = value != 0;
     else if (k == "wg_per_cu") {
       if (value < 0 || value > 64) throw std::runtime_error("setOption: wg_per_cu in [0, 64]");
       o.wg_per_cu = (int)value;
@@ -253,6 +254,7 @@ long long ma_getOption(const char* key) {
   if (k == "tile_tx") return o.tile.TX;
   if (k == "tile_ty") return o.tile.TY;
   if (k == "tile_vx") return o.tile.VX;
+  if (k == "tile_pf") return o.tile.PF;
   if (k == "wg_per_cu") return o.wg_per_cu;
   if (k == "batch_tasks") return o.batch_tasks;
   if (k == "queue") return o.queue;
@@ -303,7 +305,8 @@ const char* ma_stats(void) {
     << ", \"bytes_d2h\": " << r.cnt.bytes_d2h << ", \"store_in_use\": " << r.store.in_use()
     << ", \"store_cached\": " << r.store.cached() << ", \"graphs\": " << dev::graph_cache_size()
     << ", \"ms_last_eval\": " << r.cnt.ms_last_eval << ", \"ms_fuse\": " << r.cnt.ms_fuse
-    << ", \"ms_codegen\": " << r.cnt.ms_codegen << ", \"ms_launch\": " << r.cnt.ms_launch << "}";
+    << ", \"ms_codegen\": " << r.cnt.ms_codegen << ", \"ms_launch\": " << r.cnt.ms_launch
+    << ", \"ms_plan\": " << r.cnt.ms_plan << ", \"ms_prepare\": " << r.cnt.ms_prepare << "}";
   g_json = s.str();
   return g_json.c_str();
 }

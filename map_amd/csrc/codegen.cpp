@@ -104,9 +104,10 @@ TileShape default_tile(const Group& g) {
       minsz = std::min(minsz, sizeof_dt(n->meta.dt));
   if (minsz == 8) minsz = 4;
   TileShape t;
-  // every type moves VX = 4 cells per thread (16 bytes for FP32); 64 threads
-  // per tile row, so a wavefront covers 256 contiguous cells of a row
+  // 64 threads per tile row, each moving one vector of the narrowest type:
+  // 8 cells of 1-byte types, 4 cells of 2- and 4-byte types, 2 of 8-byte types
   if (minsz >= 8) { t.VX = 2; t.TX = 128; t.TY = 16; }
+  else if (minsz == 1) { t.VX = 8; t.TX = 512; t.TY = 16; }
   else { t.VX = 4; t.TX = 256; t.TY = 16; }
   return t;
 }
@@ -197,6 +198,10 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     ks.params.push_back(p);
   }
   for (Node* n : d2_outs) { Param p; p.kind = Param::OutD2; p.node = n; p.dt = n->meta.dt; ks.params.push_back(p); }
+  for (Node* n : body)
+    if (n->kind == Kind::Convolution && is_float_dt(n->mask.dtype())) {
+      Param p; p.kind = Param::Mask; p.node = n; p.dt = n->mask.dtype(); ks.params.push_back(p);
+    }
   // layout of P::Args: members in parameter order at their natural alignment
   // (In<T> = {const T* p; T v; int fixed}, aligned to 8)
   {
@@ -205,6 +210,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
       size_t sz = 8, al = 8;
       if (p.kind == Param::InD2) sz = sizeof_dt(p.dt) == 8 ? 24 : 16;
       if (p.kind == Param::InD0) sz = al = (size_t)sizeof_dt(p.dt);
+      if (p.kind == Param::Mask) { al = (size_t)sizeof_dt(p.dt); sz = al * p.node->mask.v.size(); }
       off = (off + al - 1) / al * al;
       p.off = off;
       p.size = sz;
@@ -261,6 +267,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     if (p.kind == Param::InD2) s << "    mx::In<" << CT(p.dt) << "> i" << LN(p.node) << ";\n";
     if (p.kind == Param::InD0) s << "    " << CT(p.dt) << " s" << LN(p.node) << ";\n";
     if (p.kind == Param::OutD2) s << "    " << CT(p.dt) << "* __restrict__ o" << LN(p.node) << ";\n";
+    if (p.kind == Param::Mask) s << "    " << CT(p.dt) << " m" << LN(p.node) << "[" << p.node->mask.v.size() << "];\n";
   }
   if (ks.params.empty()) s << "    int unused;\n";
   s << "  };\n";
@@ -291,6 +298,20 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   }
   s << "    int unused;\n  };\n";
 
+  // sin and cos of one value in one group share one sincos call (one range
+  // reduction): paired when both are emitted in the same places (halo loop
+  // and/or output cell), the first by id computes both
+  std::map<const Node*, const Node*> sc_pair;
+  for (Node* n : g.nodes) {
+    if (n->kind != Kind::Unary || n->op != SIN || !is_float_dt(n->meta.dt)) continue;
+    for (Node* m : g.nodes)
+      if (m->kind == Kind::Unary && m->op == COS && m->prev[0] == n->prev[0] && m->meta.dt == n->meta.dt &&
+          !sc_pair.count(m) && halo.count(n) == halo.count(m) && centre.count(n) == centre.count(m)) {
+        sc_pair[n] = m;
+        sc_pair[m] = n;
+        break;
+      }
+  }
   auto emit_node = [&](std::ostream& o, Node* n, bool in_pre) {
     const std::string T = CT(n->meta.dt);
     switch (n->kind) {
@@ -301,8 +322,21 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
           << (n->op == D1 ? "k.bx * k.BS0 + x" : "k.by * k.BS1 + y") << ");\n"; break;
       case Kind::Cast:
         o << "    const " << T << " " << V(n) << " = (" << T << ")" << V(n->prev[0]) << ";\n"; break;
-      case Kind::Unary:
+      case Kind::Unary: {
+        auto pr = sc_pair.find(n);
+        if (pr != sc_pair.end()) {
+          const Node* sn = n->op == SIN ? n : pr->second;
+          const Node* cn = n->op == COS ? n : pr->second;
+          if (n->id < pr->second->id) {
+            o << "    " << T << " " << V(sn) << "_s, " << V(cn) << "_c;\n";
+            o << "    " << (T == std::string("float") ? "sincosf(" : "sincos(") << V(n->prev[0]) << ", &" << V(sn) << "_s, &"
+              << V(cn) << "_c);\n";
+          }
+          o << "    const " << T << " " << V(n) << " = " << V(n) << (n->op == SIN ? "_s" : "_c") << ";\n";
+          break;
+        }
         o << "    const " << T << " " << V(n) << " = " << unary_stmt(n, V(n->prev[0]), n->prev[0]->meta.dt) << ";\n"; break;
+      }
       case Kind::Binary:
         o << "    const " << T << " " << V(n) << " = " << binary_stmt(n, V(n->prev[0]), V(n->prev[1])) << ";\n"; break;
       case Kind::Conditional:
@@ -315,11 +349,13 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
         // acc = 0; for y (outer), x (inner): acc += S[y][x] * W[y][x]  (FocalSkeleton.cpp:300-336)
         o << "    " << T << " " << V(n) << " = 0;\n";
         const int h0 = n->halo(0), h1 = n->halo(1);
+        const bool marg = is_float_dt(n->mask.dtype());  // weights from P::Args (packed %f-quantised)
         for (int i1 = -h1; i1 <= h1; i1++)
           for (int i0 = -h0; i0 <= h0; i0++) {
-            Variant w = quantize_literal(n->mask.at(i0 + h0, i1 + h1));
+            std::string w = marg ? "a.m" + LN(n) + "[" + std::to_string((i1 + h1) * n->mask.ds.arr[0] + (i0 + h0)) + "]"
+                                 : lit(quantize_literal(n->mask.at(i0 + h0, i1 + h1)));
             o << "    " << V(n) << " += W.w" << LN(n->prev[0]) << "[H1 + (" << i1 << ")][PAD + v + (" << i0 << ")] * "
-              << lit(w) << ";\n";
+              << w << ";\n";
           }
         break;
       }
@@ -349,18 +385,29 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     (void)in_pre;
   };
 
-  // pre_vec() / pre_row(): values of the staged nodes at the VX halo cells
-  // x0v..x0v+VX-1 of row y (inputs read with ldv's border rule, or through a
-  // known row source), stored to LDS as vectors
-  auto pre_body = [&](const char* fname, bool row) {
+  // pre_vec() / put_row(): values of the staged nodes at the VX halo cells
+  // x0v..x0v+VX-1 of row y, stored to LDS as vectors; pre_vec reads the
+  // inputs with ldv's border rule, put_row takes them from a Raw record that
+  // fetch_row loaded earlier through a known row source (so the loads of the
+  // next tile can be in flight while the current one is computed)
+  std::vector<Node*> raw_in;
+  for (Node* n : all)
+    if (halo.count(n) && in_set.count(n) && n->meta.nd == D2) raw_in.push_back(n);
+  s << "  template <int VX> struct Raw {\n";
+  for (Node* n : raw_in) s << "    " << CT(n->meta.dt) << " b" << LN(n) << "[VX];\n";
+  s << "    int unused;\n  };\n";
+  s << "  template <int VX> __device__ static __forceinline__ void fetch_row(const Args& a, const mx::RowSrc& rs, int x0v, Raw<VX>& r) {\n";
+  for (Node* n : raw_in)
+    s << "    mx::ldrow<" << CT(n->meta.dt) << ", VX>(a.i" << LN(n) << ", rs, x0v, r.b" << LN(n) << ");\n";
+  s << "    (void)a; (void)rs; (void)x0v; (void)r;\n  }\n";
+  auto pre_body = [&](const char* fname, bool put) {
     s << "  template <int VX> __device__ static __forceinline__ void " << fname << "(const Args& a, const mx::Blk& k, "
-      << (row ? "const mx::RowSrc& rs, " : "") << "int x0v, int y, Smem& sm, int si) {\n";
-    for (Node* n : all)
-      if (halo.count(n) && in_set.count(n) && n->meta.nd == D2) {
-        s << "    " << CT(n->meta.dt) << " b" << LN(n) << "[VX];\n";
-        if (row) s << "    mx::ldrow<" << CT(n->meta.dt) << ", VX>(a.i" << LN(n) << ", rs, x0v, b" << LN(n) << ");\n";
-        else s << "    mx::ldv<" << CT(n->meta.dt) << ", VX, H0>(a.i" << LN(n) << ", k, x0v, y, b" << LN(n) << ");\n";
-      }
+      << (put ? "const Raw<VX>& r, " : "") << "int x0v, int y, Smem& sm, int si) {\n";
+    for (Node* n : raw_in) {
+      s << "    " << CT(n->meta.dt) << " b" << LN(n) << "[VX];\n";
+      if (put) s << "#pragma unroll\n    for (int v = 0; v < VX; v++) b" << LN(n) << "[v] = r.b" << LN(n) << "[v];\n";
+      else s << "    mx::ldv<" << CT(n->meta.dt) << ", VX, H0>(a.i" << LN(n) << ", k, x0v, y, b" << LN(n) << ");\n";
+    }
     for (Node* st : staged) s << "    " << CT(st->meta.dt) << " t" << LN(st) << "[VX];\n";
     s << "#pragma unroll\n    for (int v = 0; v < VX; v++) {\n      const int x = x0v + v;\n";
     for (Node* n : all) {
@@ -376,7 +423,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     s << "    (void)a; (void)k; (void)y; (void)sm; (void)si;\n  }\n";
   };
   pre_body("pre_vec", false);
-  pre_body("pre_row", true);
+  pre_body("put_row", true);
   s << "  template <int VX> __device__ static __forceinline__ void pre_zero(Smem& sm, int si) {\n";
   for (Node* st : staged)
     s << "    { " << CT(st->meta.dt) << " z[VX] = {}; mx::stvec<" << CT(st->meta.dt) << ", VX>(&sm.sh" << LN(st) << "[si], z); }\n";
@@ -462,7 +509,8 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   // the host packs P::Args and TaskRec byte by byte: pin the layout
   s << "static_assert(sizeof(P::Args) == " << ks.args_size << ", \"Args layout\");\n";
   for (const Param& p : ks.params)
-    s << "static_assert(__builtin_offsetof(P::Args, " << (p.kind == Param::InD2 ? "i" : p.kind == Param::InD0 ? "s" : "o")
+    s << "static_assert(__builtin_offsetof(P::Args, "
+      << (p.kind == Param::InD2 ? "i" : p.kind == Param::InD0 ? "s" : p.kind == Param::Mask ? "m" : "o")
       << LN(p.node) << ") == " << p.off << ", \"Args layout\");\n";
   s << "static_assert(sizeof(mx::TaskRec<P::Args>) == " << ks.task_size << " && __builtin_offsetof(mx::TaskRec<P::Args>, L) == "
     << ks.task_launch_off << ", \"TaskRec layout\");\n";
@@ -490,7 +538,8 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   if (!zonal.empty()) s << "  __syncthreads();\n";
   s << "  __shared__ P::Smem sm;\n  __shared__ int slot;\n";
   s << "  mx::run_chunks<P, " << tile.TX << ", " << tile.TY << ", " << tile.VX << ", "
-    << (ks.form == Form::Focal ? "true" : "false") << ">(a0, L0, tasks, starts, ntasks, nchunks, queue, zpart, acc, sm, &slot);\n";
+    << (ks.form == Form::Focal ? "true" : "false") << ", " << (tile.PF ? "true" : "false")
+    << ">(a0, L0, tasks, starts, ntasks, nchunks, queue, zpart, acc, sm, &slot);\n";
   s << "}\n";
 
   // fold kernel: folds partial records in index order. Work-group (f, t) folds

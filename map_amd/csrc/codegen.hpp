@@ -17,7 +17,10 @@ enum class Form : int { Local = 0, Focal = 1 };
 
 // One member of the kernel's argument struct (P::Args), in declaration order.
 struct Param {
-  enum Kind : int { InD2, InD0, OutD2 } kind;
+  // Mask: the weights of a convolution with a floating mask, an array in
+  // P::Args (so kernels do not depend on the weights); integer masks are
+  // compiled in as literals
+  enum Kind : int { InD2, InD0, OutD2, Mask } kind;
   Node* node = nullptr;
   int dt = 0;
   size_t off = 0, size = 0;  // byte offset and size inside P::Args
@@ -35,7 +38,8 @@ struct ZonalOut {
 
 struct TileShape {
   int TX = 64, TY = 16, VX = 1;
-  bool operator==(const TileShape& o) const { return TX == o.TX && TY == o.TY && VX == o.VX; }
+  int PF = 0;  // focal groups: load the next tile's rows while computing the current one
+  bool operator==(const TileShape& o) const { return TX == o.TX && TY == o.TY && VX == o.VX && PF == o.PF; }
 };
 
 struct KernelSpec {

@@ -595,8 +595,9 @@ uint32_t morton2(uint32_t x, uint32_t y) {
 
 // Cells per chunk of a zonal group: one partial record per chunk.
 constexpr int kZonalChunkCells = 65536;
-// Tiles per chunk of other groups when a device queue hands out the chunks.
-constexpr int kQueueChunkTiles = 8;
+// Tiles per chunk of other groups: consecutive tiles of a block run on one
+// work-group (halo rows shared with the previous tile are still in L2).
+constexpr int kChunkTiles = 8;
 }  // namespace
 
 // One fused group over its raster: the unit a launch runs, alone or in a batch.
@@ -609,9 +610,9 @@ struct Runtime::Task {
   int nchunks = 0;
 
   // chunk split: zonal groups keep CH fixed (it decides their partial records)
-  void chunking(bool queue, int chunk_tiles) {
+  void chunking(int chunk_tiles) {
     if (ke->spec.zouts.empty())
-      L.CH = std::max(1, std::min(tiles, chunk_tiles > 0 ? chunk_tiles : (queue ? kQueueChunkTiles : 1)));
+      L.CH = std::max(1, std::min(tiles, chunk_tiles > 0 ? chunk_tiles : kChunkTiles));
     L.CPB = (tiles + L.CH - 1) / L.CH;
     nchunks = L.nblk * L.CPB;
   }
@@ -662,6 +663,14 @@ bool Runtime::prepare_task(const Group& g, Eval& e, Task& t) {
       // In<T>: pointer, value 0, fixed 0 (inputs are materialised rasters)
       void* d = resident_.at(p.node);
       std::memcpy(t.args.data() + p.off, &d, sizeof d);
+    } else if (p.kind == Param::Mask) {
+      // weights in mask order (row-major, y outer), with the %f quantisation of literals
+      const Mask& mk = p.node->mask;
+      const size_t es = (size_t)sizeof_dt(p.dt);
+      for (int i = 0; i < (int)mk.v.size(); i++) {
+        Variant w = variant_convert(quantize_literal(mk.v[i]), p.dt);
+        std::memcpy(t.args.data() + p.off + i * es, &w.u, es);
+      }
     } else {
       Variant v = variant_convert(p.node->value, p.dt);
       std::memcpy(t.args.data() + p.off, &v.u, (size_t)sizeof_dt(p.dt));
@@ -820,7 +829,7 @@ void Runtime::run_batch(const std::vector<Task*>& batch, std::vector<Readback>* 
 
   if (opt.form == FORM_PLAIN) {
     for (Task* t : batch) {
-      t->chunking(false, opt.chunk_tiles);
+      t->chunking(opt.chunk_tiles > 0 ? opt.chunk_tiles : 1);  // one block per launch: every tile its own chunk
       void* parts = zonal ? store.alloc((size_t)t->nchunks * ks.part_bytes) : nullptr;
       std::vector<std::pair<uint32_t, int>> order;
       for (int b = 0; b < t->L.nblk; b++) order.push_back({morton2(b % t->L.NB0, b / t->L.NB0), b});
@@ -842,7 +851,7 @@ void Runtime::run_batch(const std::vector<Task*>& batch, std::vector<Readback>* 
   if (batch.size() == 1) {
     Task* t = batch[0];
     const bool q = opt.queue == 1;
-    t->chunking(q, opt.chunk_tiles);
+    t->chunking(opt.chunk_tiles);
     void* parts = zonal ? store.alloc((size_t)t->nchunks * ks.part_bytes) : nullptr;
     launch(t->args, t->L, nul, nul, 1, t->nchunks, q ? queue() : nul, parts, grid_for(t->nchunks));
     if (zonal) {
@@ -857,7 +866,7 @@ void Runtime::run_batch(const std::vector<Task*>& batch, std::vector<Readback>* 
   std::vector<int> starts(T + 1, 0);
   int maxn = 0;
   for (int i = 0; i < T; i++) {
-    batch[i]->chunking(q, opt.chunk_tiles);
+    batch[i]->chunking(opt.chunk_tiles);
     starts[i + 1] = starts[i] + batch[i]->nchunks;
     maxn = std::max(maxn, batch[i]->nchunks);
   }
@@ -892,14 +901,17 @@ void Runtime::run_levels(const std::vector<std::pair<Eval*, size_t>>& work, std:
   std::vector<Task> tasks(n);
   std::vector<Task*> order;
   size_t i = 0;
-  for (auto& w : work)
-    for (const Group* gp : w.first->levels[w.second]) {
-      const Group& g = *gp;
-      cnt.groups++;
-      Task& t = tasks[i++];
-      if (g.numdim() == D0) { scalar_task(g); continue; }
-      if (prepare_task(g, *w.first, t)) order.push_back(&t);
-    }
+  {
+    HostTimer ht_prep(cnt.ms_prepare);
+    for (auto& w : work)
+      for (const Group* gp : w.first->levels[w.second]) {
+        const Group& g = *gp;
+        cnt.groups++;
+        Task& t = tasks[i++];
+        if (g.numdim() == D0) { scalar_task(g); continue; }
+        if (prepare_task(g, *w.first, t)) order.push_back(&t);
+      }
+  }
   // tasks of one kernel form one batch (in order of first appearance)
   std::vector<std::vector<Task*>> batches;
   std::unordered_map<std::string, size_t> at;
@@ -924,7 +936,7 @@ std::string Runtime::plan_key(const std::vector<Node*>& list, const std::vector<
   std::string k;
   k.reserve(list.size() * 9 + targets.size() * 8 + 16);
   auto put = [&](int v) { k.append((const char*)&v, sizeof v); };
-  put(opt.code_fusion); put(opt.focal_zonal); put(opt.tile.TX); put(opt.tile.TY); put(opt.tile.VX);
+  put(opt.code_fusion); put(opt.focal_zonal); put(opt.tile.TX); put(opt.tile.TY); put(opt.tile.VX); put(opt.tile.PF);
   for (Node* n : list) { put(n->id); k.push_back(n->cut ? 'c' : (n->isOutput() ? 'o' : 'n')); }
   k.push_back('|');
   for (Node* t : targets) put(t->id);
@@ -952,6 +964,7 @@ Runtime::KernelEntry& Runtime::kernel_for(Plan* plan, const Group& g, const std:
 // groups (cached by evaluation list), their levels and raster uses. Leaves
 // the cut marks set (the caller clears them with uncut()).
 void Runtime::eval_plan(Eval& e) {
+  HostTimer ht_plan(cnt.ms_plan);
   e.list = cut_list(e.targets);
   FuseOptions fo;
   fo.code_fusion = opt.code_fusion;
@@ -1104,7 +1117,23 @@ std::string Runtime::plan(const std::vector<Node*>& targets) {
           continue;
         }
         j << ", \"kernel\": \"" << ks.name << "\", \"form\": " << (int)ks.form << ", \"halo\": [" << ks.H0 << ", " << ks.H1
-          << "], \"source\": \"" << esc(ks.source) << "\"";
+          << "], \"tile\": [" << ks.tile.TX << ", " << ks.tile.TY << ", " << ks.tile.VX << ", " << ks.tile.PF
+          << "], \"tune_key\": \"" << esc(tune_key(*g)) << "\", \"source\": \"" << esc(ks.source) << "\"";
+        // floating mask weights as the kernel receives them (quantised like printed literals)
+        j << ", \"mask_args\": [";
+        bool fm = true;
+        for (const Param& p : ks.params) {
+          if (p.kind != Param::Mask) continue;
+          j << (fm ? "[" : ", [");
+          fm = false;
+          for (size_t i = 0; i < p.node->mask.v.size(); i++) {
+            char b[40];
+            std::snprintf(b, sizeof b, "%.17g", variant_as<double>(variant_convert(quantize_literal(p.node->mask.v[i]), p.dt)));
+            j << (i ? ", " : "") << b;
+          }
+          j << "]";
+        }
+        j << "]";
       }
     }
     j << "}";
@@ -1116,8 +1145,10 @@ std::string Runtime::plan(const std::vector<Node*>& targets) {
 std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   std::lock_guard<std::recursive_mutex> lk(mtx_);
   if (reps < 1) reps = 1;
-  static const TileShape cands[] = {{256, 16, 4}, {256, 32, 4}, {256, 8, 4}, {128, 32, 4}, {128, 16, 4},
-                                    {512, 16, 4}, {128, 16, 2}, {256, 16, 2}, {64, 16, 1}, {64, 32, 1}};
+  static const TileShape cands[] = {{256, 16, 4, 0}, {256, 32, 4, 0}, {512, 16, 4, 0}, {128, 32, 4, 0},
+                                    {512, 16, 8, 0}, {512, 8, 8, 0},  {256, 16, 8, 0}, {256, 32, 8, 0},
+                                    {128, 16, 2, 0}, {256, 16, 2, 0}, {64, 16, 1, 0},  {64, 32, 1, 0},
+                                    {256, 16, 4, 1}, {512, 16, 8, 1}};
   const TileShape saved = opt.tile;
   const bool saved_graph = opt.graph, saved_prof = dev::profile_on();
   opt.graph = false;
@@ -1145,7 +1176,7 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
         const double avg = k.total_ms / k.count;
         auto b = best.find(kg->second);
         if (b == best.end() || avg < b->second.first) best[kg->second] = {avg, c};
-        rep << (first ? "" : ", ") << "{\"tile\": [" << c.TX << ", " << c.TY << ", " << c.VX << "], \"kernel\": \""
+        rep << (first ? "" : ", ") << "{\"tile\": [" << c.TX << ", " << c.TY << ", " << c.VX << ", " << c.PF << "], \"kernel\": \""
             << k.name << "\", \"avg_ms\": " << avg << "}";
         first = false;
       }
@@ -1164,7 +1195,7 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   for (auto& b : best) {
     tuned_[b.first] = b.second.second;
     rep << (first ? "" : ", ") << "{\"avg_ms\": " << b.second.first << ", \"tile\": [" << b.second.second.TX << ", "
-        << b.second.second.TY << ", " << b.second.second.VX << "]}";
+        << b.second.second.TY << ", " << b.second.second.VX << ", " << b.second.second.PF << "]}";
     first = false;
   }
   rep << "]}";
@@ -1175,7 +1206,7 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   return rep.str();
 }
 
-// The tuning cache file: one line per entry, "TX TY VX <tune key>".
+// The tuning cache file: one line per entry, "TX TY VX PF <tune key>".
 void Runtime::tune_load(const std::string& path) {
   std::ifstream f(path);
   if (!f) return;
@@ -1184,7 +1215,7 @@ void Runtime::tune_load(const std::string& path) {
     std::istringstream l(line);
     TileShape t;
     std::string key;
-    if (!(l >> t.TX >> t.TY >> t.VX)) continue;
+    if (!(l >> t.TX >> t.TY >> t.VX >> t.PF)) continue;
     std::getline(l, key);
     if (!key.empty() && key[0] == ' ') key.erase(0, 1);
     if (!key.empty() && t.TX > 0 && t.VX > 0 && t.TY > 0) tuned_[key] = t;
@@ -1193,7 +1224,8 @@ void Runtime::tune_load(const std::string& path) {
 
 void Runtime::tune_save(const std::string& path) const {
   std::ofstream f(path, std::ios::trunc);
-  for (auto& e : tuned_) f << e.second.TX << " " << e.second.TY << " " << e.second.VX << " " << e.first << "\n";
+  for (auto& e : tuned_)
+    f << e.second.TX << " " << e.second.TY << " " << e.second.VX << " " << e.second.PF << " " << e.first << "\n";
 }
 
 }  // namespace mapx

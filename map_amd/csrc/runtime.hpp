@@ -32,8 +32,8 @@ struct Options {
   int wg_per_cu = 0;          // grid size cap in work-groups per CU (0: the kernel's occupancy)
   bool batch_tasks = true;    // independent groups of one kernel share one launch (task table)
   int queue = 0;              // device chunk queue: 0 for batched launches only, 1 always, 2 never
-  int chunk_tiles = 0;        // tiles per chunk of non-zonal groups (0: 8 with the queue, else 1)
-  TileShape tile{0, 0, 0};    // forced tile shape (0 = per-group default / tuned)
+  int chunk_tiles = 0;        // tiles per chunk of non-zonal groups (0: 8)
+  TileShape tile{0, 0, 0, 0}; // forced tile shape (TX 0 = per-group default / tuned)
 };
 
 struct Counters {
@@ -42,6 +42,7 @@ struct Counters {
   double ms_last_eval = 0;
   // host time per phase (the reference's Clock regions, runtime/Clock.hpp:24-27)
   double ms_fuse = 0, ms_codegen = 0, ms_launch = 0, ms_eval_host = 0;
+  double ms_plan = 0, ms_prepare = 0;  // evaluation planning (cut, plan lookup, levels); task preparation
 };
 
 // Stream-ordered caching allocator: all device work runs on one in-order

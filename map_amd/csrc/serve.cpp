@@ -35,6 +35,7 @@ struct Runtime::Request {
   double t_submit = 0, t_admit = 0;
   std::vector<Node*> computes;  // nodes this request computes
   std::vector<Node*> pinned;    // resident rasters it reads
+  Request* blocker = nullptr;   // queued: an admitted request computing a node it needs
 };
 
 struct Runtime::ServeState {
@@ -123,13 +124,21 @@ int Runtime::serveStep(int wait_us) {
   for (auto it = S.queued.begin(); it != S.queued.end();) {
     if ((int)S.active.size() >= S.max_inflight) break;
     Request& r = **it;
+    if (r.blocker) {  // planned again only once the request it waits for has retired
+      bool live = false;
+      for (auto& a : S.active) if (a.get() == r.blocker) live = true;
+      if (live) { ++it; continue; }
+      r.blocker = nullptr;
+    }
     Eval e;
     e.targets = r.ev.targets;
     eval_plan(e);
-    bool blocked = false;
-    for (Node* n : e.list)
-      if (!n->cut && S.owner.count(n)) { blocked = true; break; }
-    if (blocked) {
+    for (Node* n : e.list) {
+      if (n->cut) continue;
+      auto ow = S.owner.find(n);
+      if (ow != S.owner.end()) { r.blocker = ow->second; break; }
+    }
+    if (r.blocker) {
       eval_uncut(e);
       S.waits_owner++;
       ++it;

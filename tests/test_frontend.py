@@ -118,5 +118,27 @@ def test_tile_fits_the_block(M):
     for side, bs in [(96, 48), (1024, 1024), (64, 16)]:
         u = M.from_array(np.zeros((side, side), np.uint8), (bs, bs))
         src = M.plan(scripts.life(M, u, 1))["groups"][0]["source"]
-        tx, ty, vx = map(int, re.search(r"run_chunks<P, (\d+), (\d+), (\d+), true>", src).groups())
+        tx, ty, vx = map(int, re.search(r"run_chunks<P, (\d+), (\d+), (\d+), true, (?:true|false)>", src).groups())
         assert tx <= bs and ty <= bs and 256 % (tx // vx) == 0, (bs, tx, ty, vx)
+
+
+def test_iterated_groups_share_tune_key_and_kernel(M):
+    """Every generation of an iterated script (kept or not) tunes and compiles as one computation."""
+    u = M.from_array(np.zeros((128, 128), np.uint8), (64, 64))
+    gs = M.plan(scripts.life(M, u, 4))["groups"]
+    assert len(gs) == 4
+    assert len({g["tune_key"] for g in gs}) == 1 and len({g["kernel"] for g in gs}) == 1
+
+
+def test_float_mask_weights_do_not_change_the_kernel(M):
+    """Tiles with different floating 5x5 weights compile to one kernel (the
+    weights are kernel arguments), so they can share one batched launch."""
+    rng = np.random.default_rng(3)
+    d = M.from_array(np.zeros((100, 90), np.float32), (64, 64))
+    z = M.from_array(np.zeros((100, 90), np.int32), (64, 64))
+    names = set()
+    for _ in range(3):
+        w = (rng.integers(-64, 65, (5, 5)) / 64.0).tolist()
+        gs = M.plan(scripts.batch_tile_sums(M, d, z, w))["groups"]
+        names.add(tuple(g["kernel"] for g in gs if "kernel" in g))
+    assert len(names) == 1

@@ -294,3 +294,21 @@ def test_serve_results_same_alone_and_in_mix(G, O):
             ref = O.zstats(O.from_array(dem_np[di], bs), O.from_array(zone_np[di], bs), 64)
             assert np.array_equal(got[ids[i]][0]["count"], ref["count"])
             assert np.array_equal(got[ids[i]][0]["min"], ref["min"])
+
+
+def test_sin_cos_pair_within_4ulp(G, O):
+    """sin and cos of one value in one group share one sincos call; each
+    result stays within 4 ulp of the correctly rounded value."""
+    rng = np.random.Generator(np.random.PCG64(8))
+    parts = [rng.uniform(0, np.pi / 2, 40000), rng.uniform(-10, 10, 40000), rng.uniform(-1e3, 1e3, 40000),
+             rng.uniform(-1e-3, 1e-3, 11072)]
+    x = np.concatenate(parts).astype(np.float32).reshape(256, 512)
+    X = G.from_array(x, (128, 128))
+    s, c = G.sin(X), G.cos(X)
+    assert any("sincosf(" in g.get("source", "") for g in G.plan(s, c)["groups"])
+    G.eval(s, c)
+    ox = O.from_array(x, (128, 128))
+    for name, g, o in [("sin", s, O.sin(ox)), ("cos", c, O.cos(ox))]:
+        u = _ulps(G.to_array(g), o.val)
+        print("  paired %s: max %.2f ulp" % (name, u.max()), flush=True)
+        assert u.max() <= 4.0, (name, u.max())

@@ -53,16 +53,21 @@ def test_backend_fusion_rules_match_reference(M, ref):
 
 
 def test_mask_literals_match_reference(M, ref):
-    """Float mask weights are quantised as the reference prints them (%f)."""
+    """Float mask weights are quantised as the reference prints them (%f), and
+    integer masks are still printed into the kernel."""
     from oracle import ref as O
     for f, s in ref["float_literal"].items():
         want = np.float32(float(s))
         assert np.float32(O._quantize(float(np.float32(float(f))))) == want
         r = M.from_array(np.zeros((8, 8), np.float32), (8, 8))
         c = M.convolve(r, [[float(np.float32(float(f))), 0.0, 0.0], [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
-        src = M.plan(c)["groups"][-1]["source"]
-        lit = re.search(r"\+= W\.w\d+\[H1 \+ \(-1\)\]\[PAD \+ v \+ \(-1\)\] \* ([^;]+);", src).group(1)
-        assert np.float32(float.fromhex(lit.rstrip("f"))) == want, (f, lit)
+        g = M.plan(c)["groups"][-1]
+        # floating weights reach the kernel as arguments, quantised as the reference prints them
+        assert re.search(r"\+= W\.w\d+\[H1 \+ \(-1\)\]\[PAD \+ v \+ \(-1\)\] \* a\.m\d+\[0\];", g["source"])
+        assert np.float32(g["mask_args"][0][0]) == want, (f, g["mask_args"][0][0])
+    r = M.from_array(np.zeros((8, 8), np.int32), (8, 8))
+    src = M.plan(M.convolve(r, [[3, 0, -2], [0, 1, 0], [0, 0, 7]]))["groups"][-1]["source"]
+    assert re.search(r"\+= W\.w\d+\[H1 \+ \(-1\)\]\[PAD \+ v \+ \(1\)\] \* \(?-2\)?;", src), src
 
 
 def test_binary_spellings_match_reference(ref):
