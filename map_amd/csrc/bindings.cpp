@@ -306,7 +306,10 @@ const char* ma_stats(void) {
     << ", \"store_cached\": " << r.store.cached() << ", \"graphs\": " << dev::graph_cache_size()
     << ", \"ms_last_eval\": " << r.cnt.ms_last_eval << ", \"ms_fuse\": " << r.cnt.ms_fuse
     << ", \"ms_codegen\": " << r.cnt.ms_codegen << ", \"ms_launch\": " << r.cnt.ms_launch
-    << ", \"ms_plan\": " << r.cnt.ms_plan << ", \"ms_prepare\": " << r.cnt.ms_prepare << "}";
+    << ", \"ms_plan\": " << r.cnt.ms_plan << ", \"ms_prepare\": " << r.cnt.ms_prepare << ", \"occupancy\": {";
+  bool first = true;
+  for (auto& k : r.kernel_occ_) { s << (first ? "" : ", ") << "\"" << k.first << "\": " << k.second; first = false; }
+  s << "}, \"cus\": " << (dev::initialized() ? dev::num_cus() : 0) << "}";
   g_json = s.str();
   return g_json.c_str();
 }

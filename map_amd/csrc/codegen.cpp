@@ -1,4 +1,5 @@
 #include "codegen.hpp"
+#include "dev.hpp"
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -582,7 +583,8 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     s << "}\n";
   }
   std::string src = s.str();
-  const std::string h = hex_hash(src);
+  // named after its source and the templates it instantiates: a template change renames every kernel
+  const std::string h = hex_hash(src + dev::header_source());
   auto subst = [&](std::string t) {
     for (size_t pos; (pos = t.find("KRN_NAME")) != std::string::npos;) t.replace(pos, 8, "krn_" + h);
     return t;

@@ -102,7 +102,7 @@ int Node::halo(int i) const {
   }
 }
 
-std::string Node::signature() const {
+std::string Node::signature(bool float_weights) const {
   std::ostringstream s;
   s << kind_name(kind)[0] << kind_name(kind)[1] << meta.nd << name_dt(meta.dt);
   switch (kind) {
@@ -112,7 +112,8 @@ std::string Node::signature() const {
     case Kind::Neighbor: s << "n" << coord.arr[0] << "," << coord.arr[1]; break;
     case Kind::Convolution: case Kind::FocalFunc:
       s << "m" << op << ":" << mask.ds.arr[0] << "x" << mask.ds.arr[1] << name_dt(mask.dtype());
-      for (auto& w : mask.v) s << "," << variant_bits(w);
+      if (float_weights || !is_float_dt(mask.dtype()))
+        for (auto& w : mask.v) s << "," << variant_bits(w);
       break;
     case Kind::ZonalStats: s << "z" << nzones; break;
     default: break;
@@ -146,8 +147,9 @@ Node* Dag::add(std::unique_ptr<Node> n) {
   else n->id = id_count_++;
   Node* raw = n.get();
   for (auto* p : raw->prev) { p->next.push_back(raw); p->ref++; }  // Node::addNext (Node.cpp:83-86)
+  raw->gkey = k;
   nodes_.push_back(std::move(n));
-  gvn_.emplace(k, raw);
+  gvn_.emplace(std::move(k), raw);
   return raw;
 }
 
@@ -372,23 +374,27 @@ Node* Dag::scalar(Node* p) {
 
 void Dag::collect(const std::function<void(Node*)>& on_delete) {
   // Reverse id order: unlinking a node may free its prevs (Runtime.cpp:375-393).
+  bool any = false;
   for (auto it = nodes_.rbegin(); it != nodes_.rend(); ++it) {
     Node* n = it->get();
-    if (n->ref > 0 || n->prev.empty()) continue;
+    if (n->ref > 0) continue;
+    any = true;
     for (Node* p : n->prev) {
       auto f = std::find(p->next.begin(), p->next.end(), n);
       if (f != p->next.end()) { p->next.erase(f); p->ref--; }
     }
     n->prev.clear();
   }
+  if (!any) return;
   std::vector<std::unique_ptr<Node>> keep;
+  keep.reserve(nodes_.size());
   for (auto& n : nodes_) {
-    if (n->ref > 0) keep.push_back(std::move(n));
-    else if (on_delete) on_delete(n.get());
+    if (n->ref > 0) { keep.push_back(std::move(n)); continue; }
+    auto g = gvn_.find(n->gkey);
+    if (g != gvn_.end() && g->second == n.get()) gvn_.erase(g);
+    if (on_delete) on_delete(n.get());
   }
   nodes_.swap(keep);
-  gvn_.clear();  // kept nodes keep their prevs, so their keys are unchanged
-  for (auto& n : nodes_) gvn_.emplace(key(n.get()), n.get());
 }
 
 std::vector<Node*> list_upwards(const std::vector<Node*>& roots) {

@@ -49,6 +49,7 @@ struct Node {
   Meta meta;
   std::vector<Node*> prev, next;
   int ref = 0;  // references held by the front end (ma_increaseRef)
+  std::string gkey;  // value-numbering key (set when the node enters the DAG)
 
   // kind parameters
   int op = 0;            // Unary/Binary/Reduction enum, Index dim, Cast target
@@ -71,7 +72,9 @@ struct Node {
   bool isInput() const { return kind == Kind::Read || cut; }
   bool isOutput() const { return kind == Kind::Write || kind == Kind::Scalar; }
   int halo(int i) const;
-  std::string signature() const;  // kernel-cache signature of this node
+  // kind, type and parameters; float_weights = false leaves out floating mask
+  // weights (kernels take them as arguments)
+  std::string signature(bool float_weights = true) const;
 };
 
 // In-memory raster registered by ma_input: block-tiled (ROW+BLK) bytes, the

@@ -88,11 +88,11 @@ Meta Group::meta() const {
   return best;
 }
 
-std::string Group::signature() const {
+std::string Group::signature(bool float_weights) const {
   std::ostringstream s;
   for (Node* n : ins) s << n->meta.nd << name_dt(n->meta.dt) << ";";
   s << "|";
-  for (Node* n : nodes) s << n->signature() << ";";
+  for (Node* n : nodes) s << n->signature(float_weights) << ";";
   s << "|";
   for (Node* n : outs) s << n->meta.nd << name_dt(n->meta.dt) << ";";
   return s.str();

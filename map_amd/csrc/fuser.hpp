@@ -30,7 +30,7 @@ struct Group {
 
   Meta meta() const;  // metadata of the highest-dimensional input or node (Group.cpp:143-156)
   int numdim() const { return meta().nd; }
-  std::string signature() const;  // Group.cpp:262-272
+  std::string signature(bool float_weights = true) const;  // Group.cpp:262-272
 
   void addNode(Node* n);
   void removeNode(Node* n);

@@ -524,10 +524,11 @@ std::vector<Node*> Runtime::d2_outputs(const Group& g, const std::vector<Node*>&
 
 // Tile shapes are tuned per computation: the group's inputs and nodes (not
 // which of its nodes are kept: every generation of an iterated script shares
-// the key with its last one) and the block size.
+// the key with its last one; nor floating mask weights, which kernels take as
+// arguments) and the block size.
 std::string Runtime::tune_key(const Group& g) const {
   const Meta m = g.meta();
-  std::string sig = g.signature();
+  std::string sig = g.signature(false);
   sig = sig.substr(0, sig.rfind('|'));
   return sig + "#" + std::to_string(m.bs.arr[0]) + "x" + std::to_string(m.bs.arr[1]);
 }
@@ -815,7 +816,9 @@ void Runtime::run_batch(const std::vector<Task*>& batch, std::vector<Readback>* 
   fn.name = ks.name;
   // persistent grid: every work-group resident at once (no tail wave), a
   // multiple of 8 for the XCD-contiguous split
-  const int per_cu = opt.wg_per_cu > 0 ? std::min(opt.wg_per_cu, ke.occ) : ke.occ;
+  // work-groups per CU: the occupancy query's answer, or the option's value
+  // (more than fit at once simply queue behind the resident ones)
+  const int per_cu = opt.wg_per_cu > 0 ? opt.wg_per_cu : ke.occ;
   const int gmax = std::max(8, (dev::num_cus() * per_cu) & ~7);
   auto grid_for = [&](long work) { return (unsigned)((std::min<long>(work, gmax) + 7) & ~7L); };
   void* nul = nullptr;
@@ -955,6 +958,7 @@ Runtime::KernelEntry& Runtime::kernel_for(Plan* plan, const Group& g, const std:
   dev::Function f = dev::function(e.spec.source, e.spec.name);
   e.fn = f.fn;
   e.occ = dev::occupancy(f, 256);
+  kernel_occ_[e.spec.name] = e.occ;
   if (!e.spec.fin_name.empty()) e.fin = dev::function(e.spec.source, e.spec.fin_name).fn;
   cnt.compiles++;
   return plan->kernels.emplace(key, std::move(e)).first->second;

@@ -29,7 +29,7 @@ struct Options {
   bool graph = false;         // replay identical launch sequences from hipGraphs
   bool focal_zonal = false;   // fuse FOCAL -> ZONAL into one kernel
   bool code_fusion = true;
-  int wg_per_cu = 0;          // grid size cap in work-groups per CU (0: the kernel's occupancy)
+  int wg_per_cu = 0;          // persistent grid size in work-groups per CU (0: the kernel's occupancy)
   bool batch_tasks = true;    // independent groups of one kernel share one launch (task table)
   int queue = 0;              // device chunk queue: 0 for batched launches only, 1 always, 2 never
   int chunk_tiles = 0;        // tiles per chunk of non-zonal groups (0: 8)
@@ -185,6 +185,9 @@ class Runtime {
   std::map<std::string, TileShape> tuned_;
   std::map<std::string, std::string> kernel_group_;  // kernel name -> tune key
   std::map<std::string, std::string> kernel_src_;    // kernel name -> generated source
+ public:
+  std::map<std::string, int> kernel_occ_;            // kernel name -> resident work-groups per CU
+ private:
 
   std::unordered_map<Node*, void*> resident_;
   std::unordered_map<Node*, int> pins_;  // rasters read by admitted requests: not dropped

@@ -140,5 +140,5 @@ def test_float_mask_weights_do_not_change_the_kernel(M):
     for _ in range(3):
         w = (rng.integers(-64, 65, (5, 5)) / 64.0).tolist()
         gs = M.plan(scripts.batch_tile_sums(M, d, z, w))["groups"]
-        names.add(tuple(g["kernel"] for g in gs if "kernel" in g))
+        names.add(tuple((g["kernel"], g["tune_key"]) for g in gs if "kernel" in g))
     assert len(names) == 1
