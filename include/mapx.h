@@ -128,7 +128,7 @@ int ma_release(ma_node_t *node);
  * of one kernel in one evaluation share one launch through a device task
  * table), "queue" (device chunk queue: 0 batched launches only, 1 always,
  * 2 never), "chunk_tiles" (tiles a work-group takes at a time in groups
- * without reductions; 0: 8), "tile_pf" (focal groups load the next tile
+ * without reductions; 0: 16), "tile_pf" (focal groups load the next tile
  * while computing one). Returns 0 on success. */
 int ma_setOption(const char *key, long long value);
 long long ma_getOption(const char *key);

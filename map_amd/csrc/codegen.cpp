@@ -295,7 +295,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   for (size_t j = 0; j < ks.zouts.size(); j++) {
     const ZonalOut& z = ks.zouts[j];
     if (z.node->kind == Kind::ZonalReduc) s << "    " << CT(z.acc_dt) << " r" << j << ";\n";
-    else s << "    mx::ZoneLds<" << CT(z.dt) << "> z" << j << ";\n";
+    else s << "    mx::ZoneLds<" << CT(z.dt) << "> z" << j << ";\n    mx::ZoneRun<" << CT(z.dt) << "> q" << j << ";\n";
   }
   s << "    int unused;\n  };\n";
 
@@ -482,8 +482,8 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     if (z.node->kind == Kind::ZonalReduc)
       s << "    if (ok) acc.r" << j << " = " << red_op(z.red) << "::f(acc.r" << j << ", (" << CT(z.acc_dt) << ")" << V(src) << ");\n";
     else
-      s << "    mx::zone_step<" << CT(z.dt) << ", " << z.nzones << ">(acc.z" << j << ", ok, " << V(src) << ", (int)"
-        << V(z.node->prev[1]) << ");\n";
+      s << "    mx::zone_step<" << CT(z.dt) << ", " << z.nzones << ">(acc.z" << j << ", acc.q" << j << ", ok, " << V(src)
+        << ", (int)" << V(z.node->prev[1]) << ");\n";
   }
   s << "    (void)a; (void)k; (void)x; (void)y; (void)ok; (void)W; (void)C; (void)v; (void)acc; (void)out;\n  }\n";
   // flush(): writes one chunk's partial record and restarts the accumulators
@@ -499,6 +499,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
         << part_off[j] << "), (" << A << "*)red_scratch);\n";
       s << "    acc.r" << j << " = " << lit(reduction_neutral(z.red, z.acc_dt)) << ";\n";
     } else {
+      s << "    mx::zone_close<" << CT(z.dt) << ", " << z.nzones << ">(acc.z" << j << ", acc.q" << j << ");\n";
       s << "    mx::zone_flush<" << CT(z.dt) << ", " << z.nzones << ">(acc.z" << j << ", (mx::ZonePart<" << CT(z.dt)
         << ">*)(my + " << part_off[j] << "));\n";
       s << "    __syncthreads();\n    mx::zone_init<" << CT(z.dt) << ", " << z.nzones << ">(acc.z" << j
@@ -534,6 +535,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
       s << "  acc.z" << j << ".sum = zs" << j << "; acc.z" << j << ".mn = zn" << j << "; acc.z" << j << ".mx = zx" << j
         << "; acc.z" << j << ".cnt = zc" << j << ";\n";
       s << "  mx::zone_init<" << CT(z.dt) << ", " << z.nzones << ">(acc.z" << j << ");\n";
+      s << "  mx::zone_run_reset(acc.q" << j << ");\n";
     }
   }
   if (!zonal.empty()) s << "  __syncthreads();\n";

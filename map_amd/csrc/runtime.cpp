@@ -598,7 +598,7 @@ uint32_t morton2(uint32_t x, uint32_t y) {
 constexpr int kZonalChunkCells = 65536;
 // Tiles per chunk of other groups: consecutive tiles of a block run on one
 // work-group (halo rows shared with the previous tile are still in L2).
-constexpr int kChunkTiles = 8;
+constexpr int kChunkTiles = 16;
 }  // namespace
 
 // One fused group over its raster: the unit a launch runs, alone or in a batch.
