@@ -129,7 +129,9 @@ int ma_release(ma_node_t *node);
  * table), "queue" (device chunk queue: 0 batched launches only, 1 always,
  * 2 never), "chunk_tiles" (tiles a work-group takes at a time in groups
  * without reductions; 0: 16), "tile_pf" (focal groups load the next tile
- * while computing one). Returns 0 on success. */
+ * while computing one), "tile_st" (focal groups whose halo inputs are rasters
+ * hold their rows in registers instead of staging tiles in LDS; default 0).
+ * Returns 0 on success. */
 int ma_setOption(const char *key, long long value);
 long long ma_getOption(const char *key);
 /* Fusion groups and generated kernel sources of an evaluation, as JSON, without

@@ -227,6 +227,7 @@ int ma_setOption(const char* key, long long value) {
     else if (k == "tile_ty") o.tile.TY = (int)value;
     else if (k == "tile_vx") o.tile.VX = (int)value;
     else if (k == "tile_pf") o.tile.PF = value != 0;
+    else if (k == "tile_st") o.tile.ST = value != 0;
     else if (k == "wg_per_cu") {
       if (value < 0 || value > 64) throw std::runtime_error("setOption: wg_per_cu in [0, 64]");
       o.wg_per_cu = (int)value;
@@ -255,6 +256,7 @@ long long ma_getOption(const char* key) {
   if (k == "tile_ty") return o.tile.TY;
   if (k == "tile_vx") return o.tile.VX;
   if (k == "tile_pf") return o.tile.PF;
+  if (k == "tile_st") return o.tile.ST;
   if (k == "wg_per_cu") return o.wg_per_cu;
   if (k == "batch_tasks") return o.batch_tasks;
   if (k == "queue") return o.queue;

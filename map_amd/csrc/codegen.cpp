@@ -260,6 +260,19 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   const int VX = tile.VX;
   const int PAD = ks.H0 == 0 ? 0 : VX * ((ks.H0 + VX - 1) / VX);  // halo rounded up to whole vectors
   const int SW = tile.TX + 2 * PAD, HH = tile.TY + 2 * ks.H1, WW = VX + 2 * PAD, WR = 2 * ks.H1 + 1;
+  // streaming form: every staged node is an input raster, read per row as one
+  // dword-aligned span of VX + 2*A cells (A: the halo rounded up to 4 bytes)
+  std::map<const Node*, int> span_pad;
+  {
+    bool ok = ks.form == Form::Focal && tile.ST && !staged.empty();
+    for (Node* st : staged) {
+      const int sz = sizeof_dt(st->meta.dt);
+      if (!in_set.count(st) || st->meta.nd != D2 || (VX * sz) % 4) { ok = false; break; }
+      const int unit = sz >= 4 ? 1 : 4 / sz;
+      span_pad[st] = unit * ((ks.H0 + unit - 1) / unit);
+    }
+    ks.stream = ok;
+  }
   s << "#include \"mapx_dev.hiph\"\n";
   s << "namespace {\nstruct P {\n";
   s << "  static constexpr int H0 = " << ks.H0 << ", H1 = " << ks.H1 << ", PAD = " << PAD << ", SW = " << SW << ";\n";
@@ -281,6 +294,26 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   for (Node* st : staged) s << "    " << CT(st->meta.dt) << " w" << LN(st) << "[" << WR << "][" << WW << "];\n";
   if (staged.empty()) s << "    int unused;\n";
   s << "  };\n";
+  if (ks.stream) {
+    // one row of every staged input around the thread's VX cells
+    s << "  static constexpr int R = " << WR << ";\n";
+    s << "  template <int VX> struct Span {\n";
+    for (Node* st : staged) s << "    " << CT(st->meta.dt) << " s" << LN(st) << "[VX + " << 2 * span_pad[st] << "];\n";
+    s << "  };\n";
+    s << "  template <int VX> __device__ static __forceinline__ void span_row(const Args& a, const mx::Blk& k, int x, int y, const mx::RowSrc& rs, Span<VX>& sp) {\n";
+    for (Node* st : staged)
+      s << "    mx::ld_span<" << CT(st->meta.dt) << ", VX, " << span_pad[st] << ", H0>(a.i" << LN(st) << ", k, x, y, rs, sp.s" << LN(st) << ");\n";
+    s << "  }\n";
+    s << "  template <int VX> __device__ static __forceinline__ void span_zero(Span<VX>& sp) {\n";
+    for (Node* st : staged)
+      s << "#pragma unroll\n    for (int c = 0; c < VX + " << 2 * span_pad[st] << "; c++) sp.s" << LN(st) << "[c] = 0;\n";
+    s << "  }\n";
+    s << "  template <int VX> __device__ static __forceinline__ void span_win(const Span<VX>* rows, Win& W) {\n";
+    for (Node* st : staged)
+      s << "#pragma unroll\n    for (int r = 0; r < R; r++)\n#pragma unroll\n      for (int c = PAD - H0; c < PAD + VX + H0; c++) W.w"
+        << LN(st) << "[r][c] = rows[r].s" << LN(st) << "[c - PAD + " << span_pad[st] << "];\n";
+    s << "  }\n";
+  }
   // the thread's VX cells of every input read per cell
   s << "  struct Cells {\n";
   for (Node* n : all)
@@ -541,7 +574,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   if (!zonal.empty()) s << "  __syncthreads();\n";
   s << "  __shared__ P::Smem sm;\n  __shared__ int slot;\n";
   s << "  mx::run_chunks<P, " << tile.TX << ", " << tile.TY << ", " << tile.VX << ", "
-    << (ks.form == Form::Focal ? "true" : "false") << ", " << (tile.PF ? "true" : "false")
+    << (ks.form != Form::Focal ? "mx::LOCAL" : ks.stream ? "mx::STREAM" : "mx::STAGED") << ", " << (tile.PF ? "true" : "false")
     << ">(a0, L0, tasks, starts, ntasks, nchunks, queue, zpart, acc, sm, &slot);\n";
   s << "}\n";
 

@@ -39,13 +39,17 @@ struct ZonalOut {
 struct TileShape {
   int TX = 64, TY = 16, VX = 1;
   int PF = 0;  // focal groups: load the next tile's rows while computing the current one
-  bool operator==(const TileShape& o) const { return TX == o.TX && TY == o.TY && VX == o.VX && PF == o.PF; }
+  int ST = 0;  // focal groups that stage only inputs: rows in registers (stream_tile) instead of LDS
+  bool operator==(const TileShape& o) const {
+    return TX == o.TX && TY == o.TY && VX == o.VX && PF == o.PF && ST == o.ST;
+  }
 };
 
 struct KernelSpec {
   std::string name, fin_name;  // main kernel; zonal finalisation kernel ("" if none)
   std::string source;
   Form form = Form::Local;
+  bool stream = false;  // focal group run by stream_tile (register row window) rather than the LDS tile
   TileShape tile;
   int H0 = 0, H1 = 0;
   std::vector<Param> params;

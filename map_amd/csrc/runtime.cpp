@@ -939,7 +939,7 @@ std::string Runtime::plan_key(const std::vector<Node*>& list, const std::vector<
   std::string k;
   k.reserve(list.size() * 9 + targets.size() * 8 + 16);
   auto put = [&](int v) { k.append((const char*)&v, sizeof v); };
-  put(opt.code_fusion); put(opt.focal_zonal); put(opt.tile.TX); put(opt.tile.TY); put(opt.tile.VX); put(opt.tile.PF);
+  put(opt.code_fusion); put(opt.focal_zonal); put(opt.tile.TX); put(opt.tile.TY); put(opt.tile.VX); put(opt.tile.PF); put(opt.tile.ST);
   for (Node* n : list) { put(n->id); k.push_back(n->cut ? 'c' : (n->isOutput() ? 'o' : 'n')); }
   k.push_back('|');
   for (Node* t : targets) put(t->id);
@@ -1121,7 +1121,7 @@ std::string Runtime::plan(const std::vector<Node*>& targets) {
           continue;
         }
         j << ", \"kernel\": \"" << ks.name << "\", \"form\": " << (int)ks.form << ", \"halo\": [" << ks.H0 << ", " << ks.H1
-          << "], \"tile\": [" << ks.tile.TX << ", " << ks.tile.TY << ", " << ks.tile.VX << ", " << ks.tile.PF
+          << "], \"stream\": " << (ks.stream ? 1 : 0) << ", \"tile\": [" << ks.tile.TX << ", " << ks.tile.TY << ", " << ks.tile.VX << ", " << ks.tile.PF
           << "], \"tune_key\": \"" << esc(tune_key(*g)) << "\", \"source\": \"" << esc(ks.source) << "\"";
         // floating mask weights as the kernel receives them (quantised like printed literals)
         j << ", \"mask_args\": [";
@@ -1149,10 +1149,11 @@ std::string Runtime::plan(const std::vector<Node*>& targets) {
 std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   std::lock_guard<std::recursive_mutex> lk(mtx_);
   if (reps < 1) reps = 1;
-  static const TileShape cands[] = {{256, 16, 4, 0}, {256, 32, 4, 0}, {512, 16, 4, 0}, {128, 32, 4, 0},
-                                    {512, 16, 8, 0}, {512, 8, 8, 0},  {256, 16, 8, 0}, {256, 32, 8, 0},
-                                    {128, 16, 2, 0}, {256, 16, 2, 0}, {64, 16, 1, 0},  {64, 32, 1, 0},
-                                    {256, 16, 4, 1}, {512, 16, 8, 1}};
+  static const TileShape cands[] = {
+      {256, 16, 4, 0, 1}, {256, 32, 4, 0, 1}, {512, 16, 4, 0, 1}, {512, 16, 8, 0, 1}, {256, 32, 8, 0, 1},
+      {256, 64, 8, 0, 1}, {512, 64, 8, 0, 1}, {128, 32, 4, 0, 1}, {64, 32, 1, 0, 1},
+      {256, 16, 4, 0, 0}, {256, 32, 4, 0, 0}, {128, 32, 4, 0, 0}, {512, 16, 8, 0, 0}, {256, 32, 8, 0, 0},
+      {256, 16, 2, 0, 0}, {64, 32, 1, 0, 0}};
   const TileShape saved = opt.tile;
   const bool saved_graph = opt.graph, saved_prof = dev::profile_on();
   opt.graph = false;
@@ -1180,7 +1181,7 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
         const double avg = k.total_ms / k.count;
         auto b = best.find(kg->second);
         if (b == best.end() || avg < b->second.first) best[kg->second] = {avg, c};
-        rep << (first ? "" : ", ") << "{\"tile\": [" << c.TX << ", " << c.TY << ", " << c.VX << ", " << c.PF << "], \"kernel\": \""
+        rep << (first ? "" : ", ") << "{\"tile\": [" << c.TX << ", " << c.TY << ", " << c.VX << ", " << c.PF << ", " << c.ST << "], \"kernel\": \""
             << k.name << "\", \"avg_ms\": " << avg << "}";
         first = false;
       }
@@ -1199,7 +1200,7 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   for (auto& b : best) {
     tuned_[b.first] = b.second.second;
     rep << (first ? "" : ", ") << "{\"avg_ms\": " << b.second.first << ", \"tile\": [" << b.second.second.TX << ", "
-        << b.second.second.TY << ", " << b.second.second.VX << ", " << b.second.second.PF << "]}";
+        << b.second.second.TY << ", " << b.second.second.VX << ", " << b.second.second.PF << ", " << b.second.second.ST << "]}";
     first = false;
   }
   rep << "]}";
@@ -1210,7 +1211,7 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   return rep.str();
 }
 
-// The tuning cache file: one line per entry, "TX TY VX PF <tune key>".
+// The tuning cache file: one line per entry, "TX TY VX PF ST <tune key>".
 void Runtime::tune_load(const std::string& path) {
   std::ifstream f(path);
   if (!f) return;
@@ -1219,7 +1220,7 @@ void Runtime::tune_load(const std::string& path) {
     std::istringstream l(line);
     TileShape t;
     std::string key;
-    if (!(l >> t.TX >> t.TY >> t.VX >> t.PF)) continue;
+    if (!(l >> t.TX >> t.TY >> t.VX >> t.PF >> t.ST)) continue;
     std::getline(l, key);
     if (!key.empty() && key[0] == ' ') key.erase(0, 1);
     if (!key.empty() && t.TX > 0 && t.VX > 0 && t.TY > 0) tuned_[key] = t;
@@ -1229,7 +1230,8 @@ void Runtime::tune_load(const std::string& path) {
 void Runtime::tune_save(const std::string& path) const {
   std::ofstream f(path, std::ios::trunc);
   for (auto& e : tuned_)
-    f << e.second.TX << " " << e.second.TY << " " << e.second.VX << " " << e.second.PF << " " << e.first << "\n";
+    f << e.second.TX << " " << e.second.TY << " " << e.second.VX << " " << e.second.PF << " " << e.second.ST << " "
+      << e.first << "\n";
 }
 
 }  // namespace mapx

@@ -33,7 +33,7 @@ struct Options {
   bool batch_tasks = true;    // independent groups of one kernel share one launch (task table)
   int queue = 0;              // device chunk queue: 0 for batched launches only, 1 always, 2 never
   int chunk_tiles = 0;        // tiles per chunk of non-zonal groups (0: 16)
-  TileShape tile{0, 0, 0, 0}; // forced tile shape (TX 0 = per-group default / tuned)
+  TileShape tile{0, 0, 0, 0, 0}; // forced tile shape (TX 0 = per-group default / tuned)
 };
 
 struct Counters {

@@ -118,7 +118,7 @@ def test_tile_fits_the_block(M):
     for side, bs in [(96, 48), (1024, 1024), (64, 16)]:
         u = M.from_array(np.zeros((side, side), np.uint8), (bs, bs))
         src = M.plan(scripts.life(M, u, 1))["groups"][0]["source"]
-        tx, ty, vx = map(int, re.search(r"run_chunks<P, (\d+), (\d+), (\d+), true, (?:true|false)>", src).groups())
+        tx, ty, vx = map(int, re.search(r"run_chunks<P, (\d+), (\d+), (\d+), mx::(?:STAGED|STREAM), (?:true|false)>", src).groups())
         assert tx <= bs and ty <= bs and 256 % (tx // vx) == 0, (bs, tx, ty, vx)
 
 

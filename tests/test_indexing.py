@@ -116,7 +116,7 @@ def test_focal_reads_in_bounds(ds, bs, h0, h1, tile):
                 if body:
                     if y >= e1 + h1 or x >= e0 + h0:
                         continue
-                    if body_fast:  # row_src: the row's block (central, above/below or mirrored row)
+                    if body_fast or x + vx <= e0:  # row_src: the row's block (central, above/below or mirrored row)
                         assert 0 <= x and x + vx <= e0
                         if 0 <= y < e1:
                             blk, yy = b, y
@@ -256,3 +256,71 @@ def test_task_lookup_in_batched_launch():
     for c in sorted(rng.choice(total, 200, replace=False)):  # queue order: increasing per work-group
         t = task_of(starts, int(c), t)
         assert t == want[c]
+
+
+def span_pad(h0, size):
+    unit = 1 if size >= 4 else 4 // size
+    return unit * ((h0 + unit - 1) // unit)
+
+
+@pytest.mark.parametrize("ds,bs,h0,h1,tile,size", [((512, 384), (128, 128), 1, 1, (512, 16, 8), 1),
+                                                   ((200, 150), (64, 32), 1, 1, (512, 16, 8), 1),
+                                                   ((300, 220), (128, 64), 1, 1, (128, 32, 4), 4),
+                                                   ((70, 41), (32, 16), 2, 2, (128, 32, 4), 4),
+                                                   ((101, 57), (52, 27), 2, 2, (256, 16, 4), 4),
+                                                   ((98, 60), (48, 20), 1, 1, (256, 32, 8), 1)])
+def test_stream_reads_in_bounds(ds, bs, h0, h1, tile, size):
+    """Emulates the global reads of stream_tile: every row a thread fetches
+    (central rows directly, others through the row source or zeros) and its
+    span (one dword run inside the central block's columns, else own cells +
+    border-rule halo cells) stay inside the raster and on valid cells."""
+    tx_, ty_, vx = tile_for(bs, *tile)
+    a = span_pad(h0, size)
+    if (vx * size) % 4:
+        return  # the clamped tile is not eligible for the streaming form (codegen stages it in LDS)
+    assert (a * size) % 4 == 0
+    twx = tx_ // vx
+    twy = 256 // twx
+    ry_n = ty_ // twy
+    nb = ((ds[0] + bs[0] - 1) // bs[0], (ds[1] + bs[1] - 1) // bs[1])
+    total = nb[0] * nb[1] * bs[0] * bs[1]
+    tb = ((bs[0] + tx_ - 1) // tx_, (bs[1] + ty_ - 1) // ty_)
+    for b in range(nb[0] * nb[1]):
+        bx, by = b % nb[0], b // nb[0]
+        e0 = min(bs[0], ds[0] - bx * bs[0])
+        e1 = min(bs[1], ds[1] - by * bs[1])
+        for t in range(tb[0] * tb[1]):
+            x0, y0 = (t % tb[0]) * tx_, (t // tb[0]) * ty_
+            if x0 >= e0 or y0 >= e1:
+                continue
+            for tid in range(256):
+                x = x0 + (tid % twx) * vx
+                ys = y0 + (tid // twx) * ry_n
+                if x >= e0:
+                    continue  # fetches nothing (zeros), still runs the loop
+                rows = list(range(ys - h1, ys + ry_n + h1))
+                for y in rows:
+                    if y >= e1 + h1:
+                        continue  # zeros
+                    if 0 <= y < e1:
+                        blk, yy = b, y
+                    elif 0 <= by + (-1 if y < 0 else 1) < nb[1]:
+                        blk, yy = (by + (-1 if y < 0 else 1)) * nb[0] + bx, modulo(y, bs[1])
+                    else:
+                        blk, yy = b, invert(y, e1)
+                    be1 = min(bs[1], ds[1] - (blk // nb[0]) * bs[1])
+                    assert 0 <= yy < be1
+                    if x - a >= 0 and x + vx + a <= e0:
+                        cells = [(blk, c, yy) for c in range(x - a, x + vx + a)]
+                    else:
+                        cells = []
+                        for xc in range(x - a, x + vx + a):
+                            if 0 <= xc < e0:
+                                cells.append((blk, xc, yy))
+                            elif -h0 <= xc < e0 + h0:
+                                cells.append(ldf_index(ds, bs, nb, bx, by, xc, y))
+                    for cb, cx, cy in cells:
+                        be0 = min(bs[0], ds[0] - (cb % nb[0]) * bs[0])
+                        cbe1 = min(bs[1], ds[1] - (cb // nb[0]) * bs[1])
+                        assert 0 <= cx < be0 and 0 <= cy < cbe1, (b, x, y, cx, cy)
+                        assert 0 <= (cb * bs[1] + cy) * bs[0] + cx < total
