@@ -597,8 +597,14 @@ uint32_t morton2(uint32_t x, uint32_t y) {
 // Cells per chunk of a zonal group: one partial record per chunk.
 constexpr int kZonalChunkCells = 65536;
 // Tiles per chunk of other groups: consecutive tiles of a block run on one
-// work-group (halo rows shared with the previous tile are still in L2).
+// work-group (halo rows shared with the previous tile are still in L2), at
+// most kChunkTiles, fewer when the launch would otherwise leave work-groups
+// idle (at least two chunks per work-group of the grid).
 constexpr int kChunkTiles = 16;
+int adaptive_chunk(long tiles_total, int grid_max) {
+  const long per = tiles_total / (2L * std::max(1, grid_max));
+  return (int)std::max(1L, std::min<long>(kChunkTiles, per));
+}
 }  // namespace
 
 // One fused group over its raster: the unit a launch runs, alone or in a batch.
@@ -854,7 +860,7 @@ void Runtime::run_batch(const std::vector<Task*>& batch, std::vector<Readback>* 
   if (batch.size() == 1) {
     Task* t = batch[0];
     const bool q = opt.queue == 1;
-    t->chunking(opt.chunk_tiles);
+    t->chunking(opt.chunk_tiles > 0 ? opt.chunk_tiles : adaptive_chunk((long)t->tiles * t->L.nblk, gmax));
     void* parts = zonal ? store.alloc((size_t)t->nchunks * ks.part_bytes) : nullptr;
     launch(t->args, t->L, nul, nul, 1, t->nchunks, q ? queue() : nul, parts, grid_for(t->nchunks));
     if (zonal) {
@@ -868,8 +874,11 @@ void Runtime::run_batch(const std::vector<Task*>& batch, std::vector<Readback>* 
   const int T = (int)batch.size();
   std::vector<int> starts(T + 1, 0);
   int maxn = 0;
+  long tiles_total = 0;
+  for (Task* t : batch) tiles_total += (long)t->tiles * t->L.nblk;
+  const int ch = opt.chunk_tiles > 0 ? opt.chunk_tiles : adaptive_chunk(tiles_total, gmax);
   for (int i = 0; i < T; i++) {
-    batch[i]->chunking(opt.chunk_tiles);
+    batch[i]->chunking(ch);
     starts[i + 1] = starts[i] + batch[i]->nchunks;
     maxn = std::max(maxn, batch[i]->nchunks);
   }

@@ -312,3 +312,34 @@ def test_sin_cos_pair_within_4ulp(G, O):
         u = _ulps(G.to_array(g), o.val)
         print("  paired %s: max %.2f ulp" % (name, u.max()), flush=True)
         assert u.max() <= 4.0, (name, u.max())
+
+
+@pytest.mark.parametrize("st", [1])
+def test_register_window_form_matches(G, O, st):
+    """The register-window focal form (tile_st=1) gives the staged form's results:
+    Life bit-exact with the oracle on edge-block geometries, hillshade bit-exact
+    with the staged kernels, per-zone sums of a fused focal+zonal group equal."""
+    rng = np.random.Generator(np.random.PCG64(12))
+    s0 = (rng.random((150, 203)) < 0.3).astype(np.uint8)
+    dem = dem_array(300, 220, 9)
+    zones = O.voronoi(0, 0, 300, 220, rng.integers(0, 300, 12), rng.integers(0, 220, 12))
+    outs = {}
+    try:
+        for form in (0, st):
+            G.set_option("tile_st", form)
+            life = G.to_array(scripts.life(G, G.from_array(s0, (64, 32)), 7))
+            D = G.from_array(dem, (128, 64))
+            hs = G.to_array(scripts.hillshade(G, D))
+            G.set_option("focal_zonal", 1)
+            zs = G.zstats(scripts.hillshade(G, G.from_array(dem, (128, 64))), G.from_array(zones, (128, 64)), 12)
+            G.eval(zs)
+            G.set_option("focal_zonal", 0)
+            outs[form] = (life, hs, zs.table())
+    finally:
+        G.set_option("tile_st", 0)
+        G.set_option("focal_zonal", 0)
+    assert_exact(outs[st][0], O.life(s0, (64, 32), 7), "life (register window)")
+    assert_exact(outs[st][1], outs[0][1], "hillshade (register window vs staged)")
+    for k in ("count", "min", "max"):
+        assert np.array_equal(outs[st][2][k], outs[0][2][k]), k
+    assert np.allclose(outs[st][2]["sum"], outs[0][2]["sum"], rtol=1e-12, atol=1e-9)
