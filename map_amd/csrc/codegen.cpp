@@ -204,12 +204,11 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
       Param p; p.kind = Param::Mask; p.node = n; p.dt = n->mask.dtype(); ks.params.push_back(p);
     }
   // layout of P::Args: members in parameter order at their natural alignment
-  // (In<T> = {const T* p; T v; int fixed}, aligned to 8)
+  // (In<T> = {const T* p})
   {
     size_t off = 0, al_max = 1;
     for (Param& p : ks.params) {
       size_t sz = 8, al = 8;
-      if (p.kind == Param::InD2) sz = sizeof_dt(p.dt) == 8 ? 24 : 16;
       if (p.kind == Param::InD0) sz = al = (size_t)sizeof_dt(p.dt);
       if (p.kind == Param::Mask) { al = (size_t)sizeof_dt(p.dt); sz = al * p.node->mask.v.size(); }
       off = (off + al - 1) / al * al;
@@ -428,18 +427,18 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   for (Node* n : all)
     if (halo.count(n) && in_set.count(n) && n->meta.nd == D2) raw_in.push_back(n);
   s << "  template <int VX> struct Raw {\n";
-  for (Node* n : raw_in) s << "    " << CT(n->meta.dt) << " b" << LN(n) << "[VX];\n";
+  for (Node* n : raw_in) s << "    mx::Packed<" << CT(n->meta.dt) << ", VX> b" << LN(n) << ";\n";
   s << "    int unused;\n  };\n";
   s << "  template <int VX> __device__ static __forceinline__ void fetch_row(const Args& a, const mx::RowSrc& rs, int x0v, Raw<VX>& r) {\n";
   for (Node* n : raw_in)
-    s << "    mx::ldrow<" << CT(n->meta.dt) << ", VX>(a.i" << LN(n) << ", rs, x0v, r.b" << LN(n) << ");\n";
+    s << "    mx::ldpack<" << CT(n->meta.dt) << ", VX>(a.i" << LN(n) << ", rs, x0v, r.b" << LN(n) << ");\n";
   s << "    (void)a; (void)rs; (void)x0v; (void)r;\n  }\n";
   auto pre_body = [&](const char* fname, bool put) {
     s << "  template <int VX> __device__ static __forceinline__ void " << fname << "(const Args& a, const mx::Blk& k, "
       << (put ? "const Raw<VX>& r, " : "") << "int x0v, int y, Smem& sm, int si) {\n";
     for (Node* n : raw_in) {
       s << "    " << CT(n->meta.dt) << " b" << LN(n) << "[VX];\n";
-      if (put) s << "#pragma unroll\n    for (int v = 0; v < VX; v++) b" << LN(n) << "[v] = r.b" << LN(n) << "[v];\n";
+      if (put) s << "    r.b" << LN(n) << ".unpack(b" << LN(n) << ");\n";
       else s << "    mx::ldv<" << CT(n->meta.dt) << ", VX, H0>(a.i" << LN(n) << ", k, x0v, y, b" << LN(n) << ");\n";
     }
     for (Node* st : staged) s << "    " << CT(st->meta.dt) << " t" << LN(st) << "[VX];\n";
@@ -473,8 +472,8 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   for (Node* on : d2_outs) {
     const std::string T = CT(on->meta.dt), L = LN(on);
     s << "    {\n      " << T << " t[VX];\n#pragma unroll\n      for (int v = 0; v < VX; v++) t[v] = o[v].o" << L << ";\n"
-      << "      if (nv == VX) mx::stvec<" << T << ", VX>(a.o" << L << " + off, t);\n"
-      << "      else for (int v = 0; v < nv; v++) a.o" << L << "[off + v] = t[v];\n    }\n";
+      << "      if (nv == VX) mx::gstv<" << T << ", VX>(a.o" << L << " + off, t);\n"
+      << "      else for (int v = 0; v < nv; v++) mx::gst(a.o" << L << " + off + v, t[v]);\n    }\n";
   }
   s << "    (void)a; (void)off; (void)nv; (void)o;\n  }\n";
   // load_win(): the thread's window rows from LDS with aligned vector reads

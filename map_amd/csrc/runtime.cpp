@@ -667,7 +667,7 @@ bool Runtime::prepare_task(const Group& g, Eval& e, Task& t) {
   t.args.assign(ks.args_size, 0);
   for (const Param& p : ks.params) {
     if (p.kind == Param::InD2 || p.kind == Param::OutD2) {
-      // In<T>: pointer, value 0, fixed 0 (inputs are materialised rasters)
+      // In<T>: the raster's base pointer (inputs are materialised rasters)
       void* d = resident_.at(p.node);
       std::memcpy(t.args.data() + p.off, &d, sizeof d);
     } else if (p.kind == Param::Mask) {
@@ -1158,11 +1158,15 @@ std::string Runtime::plan(const std::vector<Node*>& targets) {
 std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   std::lock_guard<std::recursive_mutex> lk(mtx_);
   if (reps < 1) reps = 1;
+  // {TX, TY, VX, PF, ST}: register-window (ST), LDS-staged, LDS-staged with prefetch (PF)
   static const TileShape cands[] = {
-      {256, 16, 4, 0, 1}, {256, 32, 4, 0, 1}, {512, 16, 4, 0, 1}, {512, 16, 8, 0, 1}, {256, 32, 8, 0, 1},
-      {256, 64, 8, 0, 1}, {512, 64, 8, 0, 1}, {128, 32, 4, 0, 1}, {64, 32, 1, 0, 1},
-      {256, 16, 4, 0, 0}, {256, 32, 4, 0, 0}, {128, 32, 4, 0, 0}, {512, 16, 8, 0, 0}, {256, 32, 8, 0, 0},
-      {256, 16, 2, 0, 0}, {64, 32, 1, 0, 0}};
+      {256, 16, 4, 0, 1}, {256, 32, 4, 0, 1}, {512, 16, 8, 0, 1}, {256, 32, 8, 0, 1}, {128, 32, 4, 0, 1},
+      {64, 32, 1, 0, 1},
+      {256, 16, 4, 0, 0}, {256, 32, 4, 0, 0}, {128, 32, 4, 0, 0}, {512, 16, 4, 0, 0}, {512, 16, 8, 0, 0},
+      {256, 32, 8, 0, 0}, {512, 32, 8, 0, 0}, {256, 64, 8, 0, 0}, {1024, 16, 8, 0, 0}, {256, 16, 2, 0, 0},
+      {64, 32, 1, 0, 0},
+      {256, 32, 4, 1, 0}, {512, 16, 4, 1, 0}, {512, 32, 8, 1, 0}, {1024, 16, 8, 1, 0}, {256, 64, 8, 1, 0},
+      {256, 16, 2, 1, 0}};
   const TileShape saved = opt.tile;
   const bool saved_graph = opt.graph, saved_prof = dev::profile_on();
   opt.graph = false;

@@ -2,7 +2,8 @@
 
 Emulates, for small raster geometries, every global access the focal kernel
 body makes (device/mapx_dev.hiph: tiles of each block, body vectors through
-row_src, pad vectors through ldv's border rule inside [-H, E+H), per-thread
+row_src from clamped coordinates, vectors outside the central block's
+columns through ldv's border rule inside [-H, E+H), per-thread
 cell vectors and output stores)
 and checks that each lands inside the raster allocation and inside the block
 it claims. Geometries include tiles wider than blocks and partial edge
@@ -35,6 +36,18 @@ def ldf_index(ds, bs, nb, bx, by, x, y):
             x, y = modulo(x, bs[0]), modulo(y, bs[1])
             blk = nby * nb[0] + nbx
     return blk, x, y
+
+
+def row_src_offset(bs, nb, bx, by, e1, y):
+    """Element offset of halo row y as mx::row_src computes it: k.off + (s*y + d) * BS0."""
+    has_up, has_dn = by > 0, by + 1 < nb[1]
+    du = bs[1] * (1 - nb[0]) if has_up else -1
+    dd = bs[1] * (nb[0] - 1) if has_dn else 2 * e1 - 1
+    up, dn = y < 0, y >= e1
+    flip = (up and not has_up) or (dn and not has_dn)
+    d = du if up else (dd if dn else 0)
+    koff = (by * nb[0] + bx) * bs[0] * bs[1]
+    return koff + ((-y if flip else y) + d) * bs[0]
 
 
 def tile_for(bs, tx, ty, vx):
@@ -106,41 +119,54 @@ def test_focal_reads_in_bounds(ds, bs, h0, h1, tile):
             x0, y0 = (t % tb[0]) * tx, (t // tb[0]) * ty
             if x0 >= e0 or y0 >= e1:
                 continue
-            body_fast = x0 + tx <= e0
             pv = pad // vx
-            vecs = [(r, x0 + j * vx, True) for r in range(hh) for j in range(tx // vx)]
-            vecs += [(r, x0 - pad + j * vx if j < pv else x0 + tx + (j - pv) * vx, False)
-                     for r in range(hh) for j in range(2 * pv)]
-            for r, x, body in vecs:
-                y = y0 - h1 + r
-                if body:
-                    if y >= e1 + h1 or x >= e0 + h0:
-                        continue
-                    if body_fast or x + vx <= e0:  # row_src: the row's block (central, above/below or mirrored row)
-                        assert 0 <= x and x + vx <= e0
-                        if 0 <= y < e1:
-                            blk, yy = b, y
-                        elif 0 <= by + (-1 if y < 0 else 1) < nb[1]:
-                            blk, yy = (by + (-1 if y < 0 else 1)) * nb[0] + bx, modulo(y, bs[1])
-                        else:
-                            blk, yy = b, invert(y, e1)
-                        be1 = min(bs[1], ds[1] - (blk // nb[0]) * bs[1])
-                        assert 0 <= yy < be1
-                        reads = [(blk, x + v, yy, False) for v in range(vx)]
+            nbv = tx // vx
+            # fetch: every slot loads one vector through row_src from clamped
+            # coordinates (rows to e1+h1-1, columns to [0, bs0-vx]), also slots
+            # past the tile's vector count (they repeat its last vector)
+            fetched = {}
+            for kind, count, nper in (("body", hh * nbv, nbv), ("pad", hh * 2 * pv, 2 * pv)):
+                if count == 0:
+                    continue
+                for e in range(0, ((count + 255) // 256) * 256):
+                    if (e & ~63) >= count:
+                        continue  # the whole wavefront is past the count: no load
+                    ec = min(e, count - 1)
+                    r, j = ec // nper, ec % nper
+                    if kind == "body":
+                        x = x0 + j * vx
                     else:
-                        reads = ldv_reads(ds, bs, nb, bx, by, x, y, vx, h0)
+                        x = x0 - pad + j * vx if j < pv else x0 + tx + (j - pv) * vx
+                    y = min(y0 - h1 + r, e1 + h1 - 1)
+                    xc = max(0, min(x, bs[0] - vx))
+                    off = row_src_offset(bs, nb, bx, by, e1, y) + xc
+                    assert off % vx == 0 and 0 <= off and off + vx <= total, (b, t, kind, e)
+                    blk = off // (bs[0] * bs[1])
+                    assert blk == (off + vx - 1) // (bs[0] * bs[1]), "vector crosses a block"
+                    fetched[(kind, r, j)] = off
+            vecs = [("body", r, j, x0 + j * vx) for r in range(hh) for j in range(nbv)]
+            vecs += [("pad", r, j, x0 - pad + j * vx if j < pv else x0 + tx + (j - pv) * vx)
+                     for r in range(hh) for j in range(2 * pv)]
+            for kind, r, j, x in vecs:
+                y = y0 - h1 + r
+                if y >= e1 + h1 or x + vx <= -h0 or x >= e0 + h0:
+                    continue  # zero
+                if 0 <= x and x + vx <= e0:
+                    # the fetched word: the row's block (central, above/below) or mirrored row
+                    if 0 <= y < e1:
+                        blk, yy = b, y
+                    elif 0 <= by + (-1 if y < 0 else 1) < nb[1]:
+                        blk, yy = (by + (-1 if y < 0 else 1)) * nb[0] + bx, modulo(y, bs[1])
+                    else:
+                        blk, yy = b, invert(y, e1)
+                    be1 = min(bs[1], ds[1] - (blk // nb[0]) * bs[1])
+                    assert 0 <= yy < be1
+                    assert fetched[(kind, r, j)] == (blk * bs[1] + yy) * bs[0] + x, (b, t, kind, r, j)
+                    for v in range(vx):  # the cells match the per-cell border rule
+                        assert ldf_index(ds, bs, nb, bx, by, x + v, y) == (blk, x + v, yy)
+                    reads = [(blk, x + v, yy, False) for v in range(vx)]
                 else:
-                    if not (y < e1 + h1 and x + vx > -h0 and x < e0 + h0):
-                        continue
                     reads = ldv_reads(ds, bs, nb, bx, by, x, y, vx, h0)
-                for blk, sx, sy, vec in reads:
-                    assert 0 <= sx < bs[0] and 0 <= sy < bs[1], (b, t, x, y, sx, sy)
-                    idx = (blk * bs[1] + sy) * bs[0] + sx
-                    assert 0 <= idx < total
-                    if not vec:  # border-rule and row-source reads land on valid data
-                        be0 = min(bs[0], ds[0] - (blk % nb[0]) * bs[0])
-                        be1 = min(bs[1], ds[1] - (blk // nb[0]) * bs[1])
-                        assert sx < be0 and sy < be1, "read of padding at %s" % ((b, x, y),)
             # LDS window reads of every thread stay inside the halo tile
             for tid in range(256):
                 ly, lx = tid // twx, (tid % twx) * vx
