@@ -68,6 +68,11 @@ std::string binary_stmt(const Node* n, const std::string& l, const std::string& 
     std::string f = t == F32 ? "fmodf" : "fmod";
     return std::string("(") + CT(dt) + ")" + f + "((" + CT(t) + ")" + l + ", (" + CT(t) + ")" + r + ")";
   }
+  if (op == DIV && n->prev[1]->meta.nd == D0 && (is_float_dt(ldt) || is_float_dt(rdt))) {
+    // floating division by a work-group-uniform divisor (C++ operand conversions)
+    const int t = (ldt == F64 || rdt == F64) ? F64 : F32;
+    return std::string("(") + CT(dt) + ")mx::div_uniform((" + CT(t) + ")" + l + ", (" + CT(t) + ")" + r + ")";
+  }
   if (binary_is_operator(op)) return std::string("(") + CT(dt) + ")(" + l + " " + binary_code(op) + " " + r + ")";
   std::string L = std::string("((") + CT(dt) + ")" + l + ")", R = std::string("((") + CT(dt) + ")" + r + ")";
   if (op == MAX2 || op == MIN2) {
