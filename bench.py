@@ -373,6 +373,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-gens", type=int, default=8, help="generations timed once in the plain form")
     ap.add_argument("--tune", type=int, default=1, help="autotune the tile shape before timing")
+    ap.add_argument("--tile", default="", help="TX,TY,VX[,PF] tile shape of the Life kernel (implies --tune 0)")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the Life headline (skip hillshade, hillshade+zonal, batched tiles, serving)")
     ap.add_argument("--only", choices=["hillshade", "hillshade_zonal", "batched_tiles", "serve"],
@@ -407,6 +408,11 @@ def main():
         return
 
     tuned = None
+    if args.tile:
+        tv = [int(t) for t in args.tile.split(",")]
+        for key, val in zip(("tile_tx", "tile_ty", "tile_vx", "tile_pf"), tv):
+            M.set_option(key, val)
+        args.tune = 0
     if args.tune:
         # the tile shape is tuned on a 10-generation script over the same raster:
         # every generation is the same fused group, so the choice carries over
@@ -452,8 +458,8 @@ def main():
     try:
         with open(os.path.join(ROOT, "profiles", "life_hbm_traffic.json")) as f:
             tr = json.load(f)
-        if tr.get("kernel") == kname:
-            traffic = tr["bytes_per_launch"]
+        if kname in tr.get("kernels", {}):
+            traffic = tr["kernels"][kname]["bytes_per_launch"]
     except (OSError, ValueError, KeyError):
         pass
     launches = krec["count"]

@@ -17,6 +17,7 @@
 #include <chrono>
 #include <deque>
 #include <sstream>
+#include <algorithm>
 #include <stdexcept>
 #include <thread>
 
@@ -35,15 +36,18 @@ struct Runtime::Request {
   double t_submit = 0, t_admit = 0;
   std::vector<Node*> computes;  // nodes this request computes
   std::vector<Node*> pinned;    // resident rasters it reads
-  Request* blocker = nullptr;   // queued: an admitted request computing a node it needs
+  Request* blocker = nullptr;   // waiting: the admitted request computing a node it needs
 };
 
 struct Runtime::ServeState {
   size_t budget = 0, inflight = 0;
   int max_inflight = 0;
   long long next_id = 0;
-  std::deque<std::unique_ptr<Request>> queued;
+  std::deque<std::unique_ptr<Request>> queued;  // in arrival (id) order
   std::vector<std::unique_ptr<Request>> active;
+  // requests that need a node an admitted request computes, by that request;
+  // they return to `queued` when it retires (so admission never rescans them)
+  std::unordered_map<Request*, std::vector<std::unique_ptr<Request>>> waiting;
   struct Step {
     void* ev = nullptr;
     long id = 0;
@@ -106,6 +110,16 @@ void Runtime::serve_retire() {
     S.inflight -= r.footprint;
     S.done.push_back({r.id, r.t_submit, r.t_admit, t});
     S.completed++;
+    auto w = S.waiting.find(&r);
+    if (w != S.waiting.end()) {  // back into the queue at their arrival positions
+      for (auto& q : w->second) {
+        q->blocker = nullptr;
+        auto pos = std::lower_bound(S.queued.begin(), S.queued.end(), q->id,
+                                    [](const std::unique_ptr<Request>& a, long long id) { return a->id < id; });
+        S.queued.insert(pos, std::move(q));
+      }
+      S.waiting.erase(w);
+    }
     S.active[i] = std::move(S.active.back());
     S.active.pop_back();
   }
@@ -124,12 +138,6 @@ int Runtime::serveStep(int wait_us) {
   for (auto it = S.queued.begin(); it != S.queued.end();) {
     if ((int)S.active.size() >= S.max_inflight) break;
     Request& r = **it;
-    if (r.blocker) {  // planned again only once the request it waits for has retired
-      bool live = false;
-      for (auto& a : S.active) if (a.get() == r.blocker) live = true;
-      if (live) { ++it; continue; }
-      r.blocker = nullptr;
-    }
     Eval e;
     e.targets = r.ev.targets;
     eval_plan(e);
@@ -138,10 +146,12 @@ int Runtime::serveStep(int wait_us) {
       auto ow = S.owner.find(n);
       if (ow != S.owner.end()) { r.blocker = ow->second; break; }
     }
-    if (r.blocker) {
+    if (r.blocker) {  // planned again once that request has retired
       eval_uncut(e);
       S.waits_owner++;
-      ++it;
+      Request* b = r.blocker;
+      S.waiting[b].push_back(std::move(*it));
+      it = S.queued.erase(it);
       continue;
     }
     size_t fp = 0;
@@ -245,8 +255,10 @@ std::string Runtime::serveStats() {
   std::lock_guard<std::recursive_mutex> lk(mtx_);
   if (!srv_) throw std::runtime_error("serveStats: no server is open");
   const ServeState& S = *srv_;
+  size_t nwait = 0;
+  for (const auto& w : S.waiting) nwait += w.second.size();
   std::ostringstream s;
-  s << "{\"queued\": " << S.queued.size() << ", \"active\": " << S.active.size() << ", \"admitted\": " << S.admitted
+  s << "{\"queued\": " << S.queued.size() << ", \"waiting\": " << nwait << ", \"active\": " << S.active.size() << ", \"admitted\": " << S.admitted
     << ", \"completed\": " << S.completed << ", \"steps\": " << S.launch_steps << ", \"group_tasks\": " << S.tasks
     << ", \"inflight_bytes\": " << S.inflight << ", \"peak_inflight_bytes\": " << S.peak_inflight
     << ", \"budget_bytes\": " << S.budget << ", \"waits_memory\": " << S.waits_memory
