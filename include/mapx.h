@@ -81,9 +81,11 @@ ma_dim2_t ma_blocksize(ma_node_t *node);
 
 /* ---- I/O --------------------------------------------------------------------- */
 
-/* front/bindings.hpp:48 ma_read: a .bin ROW+BLK raster (lazy). */
+/* front/bindings.hpp:48 ma_read: a raster file by extension (file/File.cpp:25-45): .bin ROW+BLK or
+ * tiled .tif/.tiff (TIFF or BigTIFF, one uncompressed tile per block); lazy. */
 ma_node_t *ma_read(const char *file_path);
-/* front/bindings.hpp:49 ma_write: evaluates and writes a .bin file; 0 on success. */
+/* front/bindings.hpp:49 ma_write: evaluates and writes a .bin or tiled BigTIFF (.tif/.tiff) file;
+ * 0 on success. */
 int ma_write(ma_node_t *node, const char *file_path);
 
 /* ---- operations -------------------------------------------------------------- */
@@ -119,6 +121,12 @@ int ma_zonalTable(ma_node_t *node, double *sum, double *mn, double *mx, long lon
 ma_node_t *ma_input(const void *host, size_t nbytes, ma_dim2_t ds, ma_datatype_t dt, ma_dim2_t bs);
 /* Copies the resident raster of an evaluated D2 node (block-tiled) to host. */
 int ma_fetch(ma_node_t *node, void *host, size_t nbytes);
+/* Host-only file access (no device needed): reads the block-tiled payload of a .bin or tiled
+ * .tif/.tiff raster into `host` (nbytes = the whole payload; ma_read gives its sizes), or writes
+ * one as a .bin or tiled BigTIFF file. */
+int ma_fileLoad(const char *file_path, void *host, size_t nbytes);
+int ma_fileStore(const char *file_path, const void *host, size_t nbytes, ma_dim2_t ds, ma_datatype_t dt,
+                 ma_dim2_t bs);
 /* Releases the resident raster of a node (it is recomputed if evaluated again). */
 int ma_release(ma_node_t *node);
 /* Runtime option: "form" (0 plain: one launch per group per block, 1 batched),

@@ -3,6 +3,7 @@
 // extension entry points declared in include/mapx.h.
 #include "dev.hpp"
 #include "runtime.hpp"
+#include "rasterio.hpp"
 
 namespace mapx { struct Node; }
 #define MAPX_NODE_TYPE mapx::Node
@@ -206,6 +207,34 @@ ma_node_t* ma_input(const void* host, size_t nbytes, ma_dim2_t ds, ma_datatype_t
 
 int ma_fetch(ma_node_t* node, void* host, size_t nbytes) {
   return guard([&] { R().fetch(need(node), host, nbytes); return 0; }, -1);
+}
+
+int ma_fileLoad(const char* file_path, void* host, size_t nbytes) {
+  return guard([&] {
+    RasterFile f(file_path ? file_path : "");
+    if (!host || nbytes != f.payload_bytes())
+      throw std::runtime_error("fileLoad: the buffer must hold the whole block-tiled payload (" +
+                               std::to_string(f.payload_bytes()) + " bytes)");
+    f.read(0, nbytes, host);
+    return 0;
+  }, -1);
+}
+
+int ma_fileStore(const char* file_path, const void* host, size_t nbytes, ma_dim2_t ds, ma_datatype_t dt, ma_dim2_t bs) {
+  return guard([&] {
+    Meta m;
+    m.dt = dt; m.nd = D2; m.mo = ROW + BLK; m.ds = D(ds); m.bs = D(bs);
+    if (dt <= NONE_DATATYPE || dt >= N_DATATYPE || m.ds.arr[0] <= 0 || m.ds.arr[1] <= 0 || m.bs.arr[0] <= 0 ||
+        m.bs.arr[1] <= 0)
+      throw std::runtime_error("fileStore: bad raster description");
+    RasterWriter w(file_path ? file_path : "", m);
+    if (!host || nbytes != w.payload_bytes())
+      throw std::runtime_error("fileStore: the buffer must hold the whole block-tiled payload (" +
+                               std::to_string(w.payload_bytes()) + " bytes)");
+    w.append(host, nbytes);
+    w.finish();
+    return 0;
+  }, -1);
 }
 
 int ma_release(ma_node_t* node) {

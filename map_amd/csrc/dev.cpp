@@ -357,6 +357,7 @@ bool event_done(void* e) {
   return false;
 }
 void event_destroy(void* e) { if (e) (void)hipEventDestroy((hipEvent_t)e); }
+void event_sync(void* e) { check(hipEventSynchronize((hipEvent_t)e), "hipEventSynchronize"); }
 
 Timer::Timer() {
   check(hipEventCreate((hipEvent_t*)&a), "hipEventCreate");

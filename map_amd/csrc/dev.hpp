@@ -66,6 +66,7 @@ void graph_clear();  // destroys the instantiated graphs (call with the stream i
 void* event_record();          // records a new event on the compute stream
 bool event_done(void* e);      // the stream has passed the event
 void event_destroy(void* e);
+void event_sync(void* e);      // waits until the stream has passed the event
 
 // Device timing of the compute stream with HIP events.
 struct Timer {

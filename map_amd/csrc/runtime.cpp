@@ -1,4 +1,5 @@
 #include "runtime.hpp"
+#include "rasterio.hpp"
 #include "dev.hpp"
 #include <algorithm>
 #include <chrono>
@@ -67,50 +68,13 @@ Store::~Store() {
   // process exit: the device context may already be gone; leave buffers to the driver
 }
 
-// ---- .bin file format -------------------------------------------------------------
-
-static const size_t kPage = 4096;
-
-BinHeader bin_read_header(const std::string& path) {
-  std::ifstream f(path, std::ios::binary);
-  if (!f) fail("read: cannot open " + path);
-  int32_t h[7];
-  f.read((char*)h, sizeof h);
-  if (!f) fail("read: short header in " + path);
-  BinHeader b{h[0], h[1], h[2], h[3], h[4], h[5], h[6]};
-  if (b.dt <= NONE_DATATYPE || b.dt >= N_DATATYPE) fail("read: bad data type in " + path);
-  if (b.nd != D2) fail("read: only D2 rasters are supported: " + path);
-  if (b.mo != (ROW + BLK)) fail("read: memory order must be ROW+BLK: " + path);
-  if (b.ds0 <= 0 || b.ds1 <= 0 || b.bs0 <= 0 || b.bs1 <= 0) fail("read: bad sizes in " + path);
-  return b;
-}
-
-void bin_read_payload(const std::string& path, void* dst, size_t bytes) {
-  std::ifstream f(path, std::ios::binary);
-  if (!f) fail("read: cannot open " + path);
-  f.seekg((std::streamoff)kPage);
-  f.read((char*)dst, (std::streamsize)bytes);
-  if ((size_t)f.gcount() != bytes) fail("read: short payload in " + path);
-}
-
-void bin_write(const std::string& path, const Meta& m, const void* payload, size_t bytes) {
-  std::ofstream f(path, std::ios::binary | std::ios::trunc);
-  if (!f) fail("write: cannot open " + path);
-  std::vector<char> page(kPage, 0);
-  int32_t h[7] = {m.dt, m.nd, m.ds.arr[0], m.ds.arr[1], m.mo, m.bs.arr[0], m.bs.arr[1]};
-  std::memcpy(page.data(), h, sizeof h);
-  f.write(page.data(), (std::streamsize)page.size());
-  f.write((const char*)payload, (std::streamsize)bytes);
-  if (!f) fail("write: I/O error on " + path);
-}
-
+// A .bin or tiled TIFF raster file (the header is read here, the payload when
+// the node is first needed on the device).
 Node* Dag::read(const std::string& path) {
-  BinHeader b = bin_read_header(path);
+  const Meta m = raster_file_meta(path);
   auto n = std::make_unique<Node>();
   n->kind = Kind::Read;
-  n->meta.dt = b.dt; n->meta.nd = D2; n->meta.mo = b.mo;
-  n->meta.ds = dim2(b.ds0, b.ds1);
-  n->meta.bs = dim2(b.bs0, b.bs1);
+  n->meta = m;
   n->path = path;
   return add(std::move(n));
 }
@@ -443,25 +407,51 @@ void Runtime::release_unreferenced() {
   for (Node* n : gone) drop(n);
 }
 
+// Staging chunk: files and host rasters move through two pinned buffers of
+// this size, so reading chunk k overlaps the copy of chunk k-1.
+static const size_t kStage = (size_t)32 << 20;
+
+void* Runtime::staging(int i) {
+  if (!stage_buf_[i]) stage_buf_[i] = dev::malloc_host(kStage);
+  if (stage_ev_[i]) {
+    dev::event_sync(stage_ev_[i]);
+    dev::event_destroy(stage_ev_[i]);
+    stage_ev_[i] = nullptr;
+  }
+  return stage_buf_[i];
+}
+
+void Runtime::staged(int i) { stage_ev_[i] = dev::event_record(); }
+
 void Runtime::ensure_input(Node* n) {
   if (resident_.count(n)) return;
   if (n->kind != Kind::Read) fail("input raster of node " + std::to_string(n->id) + " is not available");
   const size_t bytes = rasterBytes(n->meta);
   if (n->host && n->host->dev) fail("generated raster of node " + std::to_string(n->id) + " was released");
   PauseRecording pause;
-  void* d = store.alloc(bytes);
-  void* pin = dev::malloc_host(bytes);
+  std::unique_ptr<RasterFile> file;
+  if (!n->host) {
+    file = std::make_unique<RasterFile>(n->path);
+    const Meta& m = file->meta();
+    if (m.dt != n->meta.dt || !dim2_eq(m.ds, n->meta.ds) || !dim2_eq(m.bs, n->meta.bs) || file->payload_bytes() != bytes)
+      fail("read: " + n->path + " changed since it was opened");
+  }
+  char* d = (char*)store.alloc(bytes);
   try {
-    if (n->host) std::memcpy(pin, n->host->bytes.data(), bytes);
-    else bin_read_payload(n->path, pin, bytes);
-    dev::h2d(d, pin, bytes);
-    dev::sync();
+    // chunk k is read (or copied) into buffer k%2 while chunk k-1 is in flight to HBM
+    for (size_t off = 0, k = 0; off < bytes; off += kStage, k++) {
+      const size_t len = std::min(kStage, bytes - off);
+      void* s = staging((int)(k & 1));
+      if (n->host) std::memcpy(s, n->host->bytes.data() + off, len);
+      else file->read(off, len, s);
+      dev::h2d(d + off, s, len);
+      staged((int)(k & 1));
+    }
   } catch (...) {
-    dev::free_host(pin);
+    dev::sync();
     store.release(d);
     throw;
   }
-  dev::free_host(pin);
   cnt.bytes_h2d += (long)bytes;
   resident_[n] = d;
 }
@@ -476,15 +466,29 @@ void Runtime::fetch(Node* n, void* host, size_t bytes) {
   cnt.bytes_d2h += (long)bytes;
 }
 
+// Streams a resident raster to a .bin or BigTIFF file: chunk k+1 is copied
+// from HBM while chunk k is written.
 void Runtime::write_file(Node* w) {
   Node* p = w->prev[0];
   const size_t bytes = rasterBytes(p->meta);
-  std::vector<unsigned char> h(bytes);
   PauseRecording pause;
-  dev::d2h(h.data(), residentPtr(p), bytes);
-  dev::sync();
+  RasterWriter out(w->path, p->meta);
+  const char* src = (const char*)residentPtr(p);
+  const size_t nchunks = (bytes + kStage - 1) / kStage;
+  auto issue = [&](size_t k) {
+    const size_t off = k * kStage;
+    void* s = staging((int)(k & 1));
+    dev::d2h(s, src + off, std::min(kStage, bytes - off));
+    staged((int)(k & 1));
+  };
+  if (nchunks) issue(0);
+  for (size_t k = 0; k < nchunks; k++) {
+    if (k + 1 < nchunks) issue(k + 1);
+    dev::event_sync(stage_ev_[k & 1]);
+    out.append(stage_buf_[k & 1], std::min(kStage, bytes - k * kStage));
+  }
+  out.finish();
   cnt.bytes_d2h += (long)bytes;
-  bin_write(w->path, p->meta, h.data(), bytes);
 }
 
 std::vector<Node*> Runtime::cut_list(const std::vector<Node*>& targets) {

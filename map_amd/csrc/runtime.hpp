@@ -198,19 +198,14 @@ class Runtime {
   std::unordered_map<std::string, void*> tables_;
   size_t table_bytes_ = 0;
   void* queue_ = nullptr;
+  // two pinned staging buffers through which files and host rasters stream
+  // to and from HBM, each with the event of its last copy
+  void* stage_buf_[2] = {nullptr, nullptr};
+  void* stage_ev_[2] = {nullptr, nullptr};
+  void* staging(int i);       // buffer i, once its last copy has completed
+  void staged(int i);         // records the event of a copy just enqueued on buffer i
   std::recursive_mutex mtx_;
 };
-
-// .bin ROW+BLK file format (file/binary.cpp:221-390): a 4096-byte header of
-// little-endian int32 {data_type, num_dim, ds0, ds1, mem_order, bs0, bs1},
-// zero-padded, then the blocks in proj order (bx + by*NB0), each a full
-// BS0*BS1 row-major block (edge blocks padded).
-struct BinHeader {
-  int dt, nd, ds0, ds1, mo, bs0, bs1;
-};
-BinHeader bin_read_header(const std::string& path);
-void bin_write(const std::string& path, const Meta& m, const void* payload, size_t bytes);
-void bin_read_payload(const std::string& path, void* dst, size_t bytes);
 
 // Host evaluation of a D0 node from its (D0) prevs, with C semantics in the
 // node types (ScalarTask, runtime/task/ScalarTask.cpp:69-193).

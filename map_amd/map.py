@@ -580,6 +580,30 @@ def to_array(raster):
     return _binfile.from_blocks(flat, ds, bs)
 
 
+def file_load(path):
+    """Host-only read of a .bin or tiled .tif/.tiff raster file -> 2D numpy array of its
+    data size (no device needed)."""
+    r = Raster(_lib.ma_read(path.encode()))
+    ds, bs, dt = r.datasize(), r.blocksize(), r.datatype()
+    nb = r.numblock()
+    del r
+    flat = np.empty(nb[0] * nb[1] * bs[0] * bs[1], dtype=_NP[dt])
+    if _lib.ma_fileLoad(path.encode(), flat.ctypes.data_as(ct.c_void_p), ct.c_size_t(flat.nbytes)) != 0:
+        _err(None)
+    return _binfile.from_blocks(flat, ds, bs)
+
+
+def file_store(path, a, bs, dt=None):
+    """Host-only write of a 2D numpy array as a .bin or tiled BigTIFF (.tif/.tiff) raster file
+    with blocks (= TIFF tiles) of size bs."""
+    a = np.ascontiguousarray(a)
+    code = dt if dt is not None else _CODE[a.dtype]
+    flat = _binfile.to_blocks(a.astype(_NP[code], copy=False), bs)
+    if _lib.ma_fileStore(path.encode(), flat.ctypes.data_as(ct.c_void_p), ct.c_size_t(flat.nbytes),
+                         Array([a.shape[1], a.shape[0]]), code, Array(list(bs))) != 0:
+        _err(None)
+
+
 def release(raster):
     if _lib.ma_release(raster) != 0:
         _err(None)
@@ -755,6 +779,8 @@ _proto('ma_zonalTable', [Raster, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double
                          ct.POINTER(ct.c_longlong), ct.c_int], ct.c_int)
 _proto('ma_input', [ct.c_void_p, ct.c_size_t, Array, ct.c_int, Array], Node)
 _proto('ma_fetch', [Raster, ct.c_void_p, ct.c_size_t], ct.c_int)
+_proto('ma_fileLoad', [ct.c_char_p, ct.c_void_p, ct.c_size_t], ct.c_int)
+_proto('ma_fileStore', [ct.c_char_p, ct.c_void_p, ct.c_size_t, Array, ct.c_int, Array], ct.c_int)
 _proto('ma_release', [Raster], ct.c_int)
 _proto('ma_setOption', [ct.c_char_p, ct.c_longlong], ct.c_int)
 _proto('ma_getOption', [ct.c_char_p], ct.c_longlong)
