@@ -7,8 +7,9 @@ life.py run unchanged apart from Python 3 syntax. Calls go through ctypes to
 the C ABI declared in include/mapx.h.
 
 Differences from the reference: failures raise MapError instead of aborting
-the process, and symbolic `while` loops over rasters (python/map.py:234-327)
-are not supported (evaluating a Raster as a bool raises).
+the process, and a `while` over rasters runs as ordinary Python with the D0
+condition evaluated on the device each iteration (Raster.__bool__) instead of
+being rebuilt from the loop's source into a Loop node (python/map.py:234-327).
 Extensions: zstats/zone_table (per-zone statistics), from_array/to_array
 (in-memory rasters), gen_dem/gen_zones (synthetic inputs in HBM),
 set_option/get_option, plan, compile_plan, stats.
@@ -201,7 +202,17 @@ class Raster:
         return ins._node
 
     def __bool__(self):
-        raise MapError("symbolic while-loops over rasters are not supported by this backend")
+        """Truth value of a D0 raster, evaluated on the device, so a `while` over rasters
+        (`while zor(change): ...`) runs unmodified as ordinary Python. The reference instead
+        rebuilds the loop from its source into a Loop node (python/map.py:234-327,
+        runtime/Runtime.cpp:156-311); both read the condition back once per iteration. Each
+        iteration's body is a new lazy DAG: its kernels come from the compiled-kernel cache
+        and the previous iterate stays resident while the new one links to it. A D2 raster
+        has no truth value."""
+        if self.numdim() != D0:
+            raise MapError("the truth value of a D2 raster is ambiguous (reduce it first, e.g. zor/zmax)")
+        eval(self)
+        return bool(value(self))
 
     def __hash__(self):
         return id(self)

@@ -142,3 +142,11 @@ def test_float_mask_weights_do_not_change_the_kernel(M):
         gs = M.plan(scripts.batch_tile_sums(M, d, z, w))["groups"]
         names.add(tuple((g["kernel"], g["tune_key"]) for g in gs if "kernel" in g))
     assert len(names) == 1
+
+
+def test_d2_raster_has_no_truth_value():
+    """Only D0 rasters have a truth value (the loop condition of a `while` over rasters)."""
+    from map_amd import map as M
+    r = M.from_array(np.zeros((8, 8), np.float32), (8, 8))
+    with pytest.raises(M.MapError, match="ambiguous"):
+        bool(r)

@@ -365,3 +365,23 @@ def test_raster_files_stream_through_the_device(G, O, tmp_path):
         G.write(hs, out)
         assert_exact(G.file_load(out), got, "written ." + ext)
         del D, hs
+
+
+def test_while_loop_over_a_d0_condition(G, O):
+    """`while <D0 raster>:` runs unmodified: the condition is evaluated on the device each
+    iteration; the loop ends exactly when the zonal maximum reaches 0, and the state after
+    each iteration matches the oracle's."""
+    print("case while loop 130x90 S32, blocks 64x32", flush=True)
+    rng = np.random.default_rng(8)
+    s0 = rng.integers(0, 7, (90, 130)).astype(np.int32)
+    bs = (64, 32)
+    s = G.from_array(s0, bs)
+    o = O.from_array(s0, bs)
+    n = 0
+    while G.zmax(s) > 0:
+        s = (s > 0) * (s - 1) + 0 * G.convolve(s, [[0, 1, 0], [1, 1, 1], [0, 1, 0]])
+        o = (o > 0) * (o - 1) + 0 * O.convolve(o, [[0, 1, 0], [1, 1, 1], [0, 1, 0]])
+        n += 1
+        assert_exact(G.to_array(s), o.val, "iteration %d" % n)
+    assert n == int(s0.max())
+    assert not G.to_array(s).any()
