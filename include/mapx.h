@@ -104,7 +104,11 @@ ma_node_t *ma_neighbor(ma_node_t *node, ma_dim2_t coord);
 ma_node_t *ma_convolution(ma_node_t *node, ma_variant_t *mask_v, int mask_s, ma_dim2_t mask_ds);
 /* front/bindings.hpp:73 ma_focalFunc */
 ma_node_t *ma_focalFunc(ma_node_t *node, ma_variant_t *mask_v, int mask_s, ma_dim2_t mask_ds, ma_reduction_t type);
-/* front/bindings.hpp:76 ma_zonalReduc: whole-raster reduction to a D0 value. */
+/* front/bindings.hpp:76 ma_radialScan (front/bindings.cpp:218-221): radial scan from the cell
+ * coord = {x, y}: each cell gets type (SUM/PROD/MAX/MIN) of its input and the value its ray
+ * from coord carries in (interpolated between the two cells of the previous column); F32/F64 only. */
+ma_node_t *ma_radialScan(ma_node_t *node, ma_reduction_t type, ma_dim2_t coord);
+/* front/bindings.hpp:75 ma_zonalReduc: whole-raster reduction to a D0 value. */
 ma_node_t *ma_zonalReduc(ma_node_t *node, ma_reduction_t type);
 
 /* ---- extensions (no reference counterpart) ---------------------------------- */

@@ -15,7 +15,7 @@ const char* kind_name(Kind k) {
     case Kind::Neighbor: return "Neighbor"; case Kind::Convolution: return "Convolution";
     case Kind::FocalFunc: return "FocalFunc"; case Kind::ZonalReduc: return "ZonalReduc";
     case Kind::ZonalStats: return "ZonalStats"; case Kind::Read: return "Read"; case Kind::Write: return "Write";
-    case Kind::Scalar: return "Scalar";
+    case Kind::Scalar: return "Scalar"; case Kind::RadialScan: return "RadialScan";
   }
   return "?";
 }
@@ -90,6 +90,7 @@ int Node::pattern() const {
     case Kind::Cast: case Kind::Unary: case Kind::Binary: case Kind::Conditional: return LOCAL;
     case Kind::Neighbor: case Kind::Convolution: case Kind::FocalFunc: return FOCAL;
     case Kind::ZonalReduc: case Kind::ZonalStats: return ZONAL;
+    case Kind::RadialScan: return RADIAL;
     default: return FREE;  // Constant, Index, Read, Write, Scalar (runtime/dag/Node.cpp:98)
   }
 }
@@ -116,6 +117,7 @@ std::string Node::signature(bool float_weights) const {
         for (auto& w : mask.v) s << "," << variant_bits(w);
       break;
     case Kind::ZonalStats: s << "z" << nzones; break;
+    case Kind::RadialScan: s << "r" << op << "@" << coord.arr[0] << "," << coord.arr[1]; break;
     default: break;
   }
   return s.str();
@@ -323,6 +325,26 @@ Node* Dag::zonalReduc(Node* p, int red) {
   n->op = red;
   n->prev = {p};
   n->value = reduction_neutral(red, p->meta.dt);
+  return add(std::move(n));
+}
+
+// Radial scan from the start cell (RadialScan::Factory, runtime/dag/RadialScan.cpp:34-45): a
+// raster of the input's shape and type. Floating inputs only: the reference's
+// interpolation weight is meaningless for integers (RadiatingSkeleton's
+// linear_interpolation truncates it to 1 - dif1).
+Node* Dag::radialScan(Node* p, int red, Dim2 start) {
+  if (!p || p->meta.nd != D2) fail("radialScan: needs a D2 raster");
+  if (!is_float_dt(p->meta.dt)) fail("radialScan: needs a floating raster (F32 or F64)");
+  if (red != SUM && red != PROD && red != MAX && red != MIN) fail("radialScan: reduction must be SUM, PROD, MAX or MIN");
+  if (start.n != 2 || start.arr[0] < 0 || start.arr[1] < 0 || start.arr[0] >= p->meta.ds.arr[0] ||
+      start.arr[1] >= p->meta.ds.arr[1])
+    fail("radialScan: the start cell must lie inside the raster");
+  auto n = std::make_unique<Node>();
+  n->kind = Kind::RadialScan;
+  n->meta = p->meta;
+  n->op = red;
+  n->coord = start;
+  n->prev = {p};
   return add(std::move(n));
 }
 

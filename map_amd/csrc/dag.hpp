@@ -18,7 +18,8 @@ enum class Kind : int {
   Constant, Index, Cast, Unary, Binary, Conditional,
   Neighbor, Convolution, FocalFunc,
   ZonalReduc, ZonalStats,
-  Read, Write, Scalar
+  Read, Write, Scalar,
+  RadialScan
 };
 const char* kind_name(Kind k);
 
@@ -55,7 +56,7 @@ struct Node {
   int op = 0;            // Unary/Binary/Reduction enum, Index dim, Cast target
   Variant cnst{};        // Constant
   Mask mask;             // Convolution / FocalFunc
-  Dim2 coord = dim2_none();  // Neighbor
+  Dim2 coord = dim2_none();  // Neighbor offset; RadialScan start cell
   std::string path;      // Read / Write
   int nzones = 0;        // ZonalStats
   std::shared_ptr<HostRaster> host;  // Read of an in-memory raster
@@ -103,6 +104,7 @@ class Dag {
   Node* focalFunc(Node* p, const Mask& m, int red);
   Node* zonalReduc(Node* p, int red);
   Node* zonalStats(Node* data, Node* zone, int nzones);
+  Node* radialScan(Node* p, int red, Dim2 start);
   Node* read(const std::string& path);
   Node* input(std::shared_ptr<HostRaster> r);
   Node* write(Node* p, const std::string& path);

@@ -192,8 +192,14 @@ static bool same_geometry(const Group* a, const Group* b) {
   return dim2_eq(ma.ds, mb.ds) && dim2_eq(ma.bs, mb.bs);
 }
 
+// Radial groups run alone: their kernel (a sequential-along-the-ray recurrence,
+// device/radial.hip) has no fused per-cell statements. The reference fuses
+// LOCAL groups on either side of a RADIAL one (Pattern.cpp:71,95,149).
+static bool radial(const Group* g) { return pattern_is(g->pat, RADIAL); }
+
 bool Fuser::canPipe(Group* top, Group* bot) {  // Fusioner.cpp:87-96
   if (top == bot) return false;
+  if (radial(top) || radial(bot)) return false;
   if (!same_geometry(top, bot)) return false;
   for (Group* n : top->next)
     if (n != bot && isNext(bot, n)) return false;  // a path top -> n -> ... -> bot: fusing makes a cycle
@@ -203,6 +209,7 @@ bool Fuser::canPipe(Group* top, Group* bot) {  // Fusioner.cpp:87-96
 
 bool Fuser::canFlat(Group* left, Group* right) {  // Fusioner.cpp:98-106
   if (left == right) return false;
+  if (radial(left) || radial(right)) return false;
   if (!same_geometry(left, right)) return false;
   if (isNext(right, left) || isNext(left, right)) return false;
   return can_flat_fuse(left->pat, right->pat, opt_.focal_zonal);

@@ -13,6 +13,13 @@
 #include <stdexcept>
 
 namespace mapx {
+namespace radial {  // radial.hip
+int run(void* out, const void* in, int dt_bytes, int op, int BS0, int BS1, int NB0, int NB1, int DS0, int DS1, int s0,
+        int s1, void* stream);
+}  // namespace radial
+}  // namespace mapx
+
+namespace mapx {
 
 static void fail(const std::string& m) { throw std::runtime_error(m); }
 
@@ -639,6 +646,31 @@ std::vector<Node*> Runtime::kernel_outputs(const Group& g, const std::vector<Nod
   return kouts;
 }
 
+Node* Runtime::radial_node(const Group& g) {
+  for (Node* n : g.nodes)
+    if (n->kind == Kind::RadialScan) return n;
+  return nullptr;
+}
+
+// A radial group is its RadialScan node alone (the fuser keeps it so): the
+// scan of a resident raster, launched band by band (radial.hip) in stream
+// order with the other groups of its level.
+void Runtime::radial_task(const Group& g, Eval& e) {
+  Node* n = radial_node(g);
+  if (g.nodes.size() != 1) fail("radial scan fused with other nodes");
+  for (Node* in : g.ins)
+    if (in->meta.nd == D2) ensure_input(in);
+  bool zonal = false;
+  if (kernel_outputs(g, e.targets, zonal).empty()) return;
+  const Meta& m = n->meta;
+  void* out = store.alloc(rasterBytes(m));
+  resident_[n] = out;
+  const int op = n->op == SUM ? 0 : n->op == PROD ? 1 : n->op == MAX ? 2 : 3;
+  PauseRecording pause;
+  cnt.launches += radial::run(out, residentPtr(n->prev[0]), sizeof_dt(m.dt), op, m.bs.arr[0], m.bs.arr[1], m.nb(0),
+                              m.nb(1), m.ds.arr[0], m.ds.arr[1], n->coord.arr[0], n->coord.arr[1], dev::stream());
+}
+
 bool Runtime::prepare_task(const Group& g, Eval& e, Task& t) {
   bool has_zonal = false;
   std::vector<Node*> kouts;
@@ -925,6 +957,7 @@ void Runtime::run_levels(const std::vector<std::pair<Eval*, size_t>>& work, std:
         cnt.groups++;
         Task& t = tasks[i++];
         if (g.numdim() == D0) { scalar_task(g); continue; }
+        if (radial_node(g)) { radial_task(g, *w.first); continue; }
         if (prepare_task(g, *w.first, t)) order.push_back(&t);
       }
   }
@@ -1117,6 +1150,11 @@ std::string Runtime::plan(const std::vector<Node*>& targets) {
     j << ", \"kinds\": [";
     for (size_t i = 0; i < g->nodes.size(); i++) j << (i ? ", " : "") << "\"" << kind_name(g->nodes[i]->kind) << "\"";
     j << "]";
+    if (Node* rn = radial_node(*g)) {  // precompiled band kernel (radial.hip), no generated source
+      j << ", \"radial\": {\"reduction\": " << rn->op << ", \"start\": [" << rn->coord.arr[0] << ", " << rn->coord.arr[1]
+        << "]}}";
+      continue;
+    }
     if (g->numdim() != D0) {
       std::vector<Node*> outs = d2_outputs(*g, targets), kouts;
       for (Node* n : outs)

@@ -79,6 +79,7 @@ class Runtime {
   bool isResident(const Node* n) const { return resident_.count(const_cast<Node*>(n)) != 0; }
   void* residentPtr(Node* n);
   size_t rasterBytes(const Meta& m) const;
+  static Node* radial_node(const Group& g);  // the RadialScan node of a radial group, or null
   void fetch(Node* n, void* host, size_t bytes);  // copies a resident raster (block-tiled) to host
   void drop(Node* n);                              // releases a node's resident raster
   void adopt(Node* n, void* dev) { resident_[n] = dev; }  // raster produced in HBM by a generator
@@ -148,6 +149,7 @@ class Runtime {
   std::vector<Node*> kernel_outputs(const Group& g, const std::vector<Node*>& targets, bool& has_zonal);
   void run_levels(const std::vector<std::pair<Eval*, size_t>>& work, std::vector<Readback>* async);
   bool prepare_task(const Group& g, Eval& e, Task& t);
+  void radial_task(const Group& g, Eval& e);
   void run_batch(const std::vector<Task*>& batch, std::vector<Readback>* async);
   void fold_results(const std::vector<Task*>& batch, void* parts, void* tasks_dev, void* starts_dev, int maxn,
                     std::vector<Readback>* async);

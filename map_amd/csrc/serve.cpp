@@ -159,6 +159,14 @@ int Runtime::serveStep(int wait_us) {
     try {
       for (auto& g : e.plan->groups) {
         if (g->numdim() == D0) continue;
+        if (Node* rn = radial_node(*g)) {  // no generated kernel: its output raster
+          bool z = false;
+          if (!kernel_outputs(*g, e.targets, z).empty()) fp += rasterBytes(rn->meta);
+          for (Node* n : g->ins)
+            if (n->meta.nd == D2 && resident_.count(n)) pins.push_back(n);
+            else if (n->meta.nd == D2 && n->kind == Kind::Read) fp += rasterBytes(n->meta);
+          continue;
+        }
         bool zonal = false;
         std::vector<Node*> kouts = kernel_outputs(*g, e.targets, zonal);
         KernelEntry* ke = nullptr;

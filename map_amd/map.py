@@ -463,6 +463,23 @@ def fmax(raster, lst):
     return _focal(raster, lst, MAX)
 
 
+def rsum(raster, coord):
+    """Radial scans from the cell coord = [x, y] (python/map.py:578-588)."""
+    return Raster(_lib.ma_radialScan(raster, SUM, Array(list(coord))))
+
+
+def rprod(raster, coord):
+    return Raster(_lib.ma_radialScan(raster, PROD, Array(list(coord))))
+
+
+def rmax(raster, coord):
+    return Raster(_lib.ma_radialScan(raster, MAX, Array(list(coord))))
+
+
+def rmin(raster, coord):
+    return Raster(_lib.ma_radialScan(raster, MIN, Array(list(coord))))
+
+
 def zsum(raster):
     return Raster(_lib.ma_zonalReduc(raster, SUM))
 
@@ -785,6 +802,7 @@ _proto('ma_neighbor', [Raster, Array], Node)
 _proto('ma_convolution', [Raster, ct.POINTER(Variant), ct.c_int, Array], Node)
 _proto('ma_focalFunc', [Raster, ct.POINTER(Variant), ct.c_int, Array, ct.c_int], Node)
 _proto('ma_zonalReduc', [Raster, ct.c_int], Node)
+_proto('ma_radialScan', [Raster, ct.c_int, Array], Node)
 _proto('ma_zonalStats', [Raster, Raster, ct.c_int], Node)
 _proto('ma_zonalTable', [Raster, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double), ct.POINTER(ct.c_double),
                          ct.POINTER(ct.c_longlong), ct.c_int], ct.c_int)

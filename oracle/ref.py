@@ -467,6 +467,88 @@ def zmin(x):
     return _zred(x, MIN)
 
 
+# Octants of a radial scan in the reference's RadialCase order (runtime/Direction.hpp:68-97):
+# (first direction, second direction) as unit vectors (fx, fy, sx, sy), x east, y south.
+RADIAL_OCTANTS = [(0, -1, 1, 0), (1, 0, 0, -1), (1, 0, 0, 1), (0, 1, 1, 0),
+                  (0, 1, -1, 0), (-1, 0, 0, 1), (-1, 0, 0, -1), (0, -1, -1, 0)]
+
+
+def _reach(s, n, u):
+    return n - 1 - s if u > 0 else s
+
+
+def _radial(x, red, coord):
+    """Radial scan from the cell coord = (x, y) (RadialScan, runtime/skeleton/RadiatingSkeleton.cpp:
+    188-247 and the radial helpers of runtime/skeleton/util.cpp:543-616), as the recurrence that
+    kernel computes per octant (first direction F, second S; cells with d0 = (p-s).F >= d1 =
+    (p-s).S >= 0), column d0 by column:
+      base = neutral at the start cell; R(p-F) on the F axis (d1 = 0); R(p-F-S) on the diagonal
+      (d1 = d0); otherwise a*w + b*(1-w) with a = R(p-F), b = R(p-F-S),
+      w = d1/d0*(d0-1) - (d1-1) (linear_interpolation, util.cpp:564);
+      R(p) = OP(base, in(p)) when in(p) is finite, else base (RadiatingSkeleton.cpp:305-316).
+    Every operation rounds in the raster's type (numpy element-wise ops). Cells on an axis or a
+    diagonal are computed by both of their octants; the two results are checked to be equal."""
+    v = np.asarray(x.val)
+    T = v.dtype.type
+    H, W = v.shape
+    s0, s1 = int(coord[0]), int(coord[1])
+    if not (0 <= s0 < W and 0 <= s1 < H):
+        raise ValueError("radial: start cell outside the raster")
+    neutral = {SUM: 0.0, PROD: 1.0, MAX: -np.inf, MIN: np.inf}[red]
+    op = {SUM: np.add, PROD: np.multiply, MAX: np.fmax, MIN: np.fmin}[red]
+    out = np.empty_like(v)
+    seen = np.zeros(v.shape, bool)
+    one = T(1)
+    for fx, fy, sx, sy in RADIAL_OCTANTS:
+        L0 = _reach(s0, W, fx) if fx else _reach(s1, H, fy)
+        L1 = _reach(s0, W, sx) if sx else _reach(s1, H, sy)
+        prev = None
+        for c in range(L0 + 1):
+            n = min(c, L1) + 1
+            r = np.arange(n)
+            xs, ys = s0 + c * fx + r * sx, s1 + c * fy + r * sy
+            xin = v[ys, xs]
+            base = np.empty(n, T)
+            if c == 0:
+                base[0] = T(neutral)
+            else:
+                base[0] = prev[0]
+                m = min(c - 1, L1)  # rows 0 < d1 < d0 with both predecessors in column c-1
+                if m >= 1:
+                    d1 = np.arange(1, m + 1)
+                    w = d1.astype(T) / T(c) * T(c - 1) - (d1 - 1).astype(T)
+                    base[1:m + 1] = prev[1:m + 1] * w + prev[0:m] * (one - w)
+                if c <= L1:
+                    base[c] = prev[c - 1]
+            with np.errstate(all="ignore"):
+                val = np.where(np.isfinite(xin), op(base, xin), base).astype(T)
+            old = seen[ys, xs]
+            if old.any():
+                a, b = out[ys, xs][old], val[old]
+                if not np.array_equal(a.view(np.uint8), b.view(np.uint8)):
+                    raise AssertionError("radial: octants disagree on a shared cell")
+            out[ys, xs] = val
+            seen[ys, xs] = True
+            prev = val
+    return R(out, x.dt, x.bs)
+
+
+def rsum(x, coord):
+    return _radial(x, SUM, coord)
+
+
+def rprod(x, coord):
+    return _radial(x, PROD, coord)
+
+
+def rmax(x, coord):
+    return _radial(x, MAX, coord)
+
+
+def rmin(x, coord):
+    return _radial(x, MIN, coord)
+
+
 def zstats(x, zones, nz):
     """Per-zone (sum FP64, min, max, count, mean) of x over integer zones."""
     v = np.asarray(x.val).astype(np.float64).reshape(-1)

@@ -42,8 +42,13 @@ def assert_fp32(got, ref64, scale=None, what=""):
 
 
 def assert_exact(got, ref, what=""):
+    """Equal cell for cell. A NaN matches any NaN: IEEE 754 leaves the sign and payload of the
+    NaN an invalid operation returns to the platform (x86 sets the sign bit, the GPU does not)."""
     got, ref = np.asarray(got), np.asarray(ref)
     assert got.shape == ref.shape, (got.shape, ref.shape)
-    bad = np.argwhere(got != ref)
+    diff = got != ref
+    if np.issubdtype(got.dtype, np.floating) and np.issubdtype(ref.dtype, np.floating):
+        diff &= ~(np.isnan(got) & np.isnan(ref))
+    bad = np.argwhere(diff)
     assert bad.size == 0, "%s: %d cells differ, first at %s: got %s want %s" % (
         what, len(bad), bad[0], got[tuple(bad[0])], ref[tuple(bad[0])])

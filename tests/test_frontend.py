@@ -150,3 +150,18 @@ def test_d2_raster_has_no_truth_value():
     r = M.from_array(np.zeros((8, 8), np.float32), (8, 8))
     with pytest.raises(M.MapError, match="ambiguous"):
         bool(r)
+
+
+def test_radial_scan_front_end_checks():
+    """Radial scans take floating rasters and a start cell inside the raster; the plan shows
+    a radial group by itself (its locals are not fused into it)."""
+    from map_amd import map as M
+    f = M.from_array(np.zeros((40, 60), np.float32), (32, 32))
+    with pytest.raises(M.MapError, match="floating"):
+        M.rmax(M.from_array(np.zeros((40, 60), np.int32), (32, 32)), [1, 1])
+    with pytest.raises(M.MapError, match="inside"):
+        M.rmax(f, [60, 3])
+    v = (M.rmax(f * 2.0, [10, 20]) + 1.0) * f
+    groups = M.plan(v)["groups"]
+    radial = [g for g in groups if "radial" in g]
+    assert len(radial) == 1 and radial[0]["kinds"] == ["RadialScan"] and radial[0]["radial"]["start"] == [10, 20]

@@ -385,3 +385,43 @@ def test_while_loop_over_a_d0_condition(G, O):
         assert_exact(G.to_array(s), o.val, "iteration %d" % n)
     assert n == int(s0.max())
     assert not G.to_array(s).any()
+
+
+@pytest.mark.parametrize("w,h,bs,dt", [(300, 220, (128, 64), np.float32), (257, 129, (64, 64), np.float64),
+                                       (700, 90, (256, 32), np.float32)])
+def test_radial_scans_match_oracle(G, O, w, h, bs, dt):
+    """rsum / rprod / rmax / rmin from start cells in corners, on edges, on block seams and
+    inside: bit-exact against the oracle, inputs with NaN and infinities sprinkled in."""
+    print("case radial", w, h, bs, np.dtype(dt).name, flush=True)
+    rng = np.random.default_rng(w + h)
+    a = (rng.standard_normal((h, w)) * 3).astype(dt)
+    a[rng.random((h, w)) < 0.02] = np.nan
+    a[rng.random((h, w)) < 0.01] = np.inf
+    p = np.where(np.isfinite(a), 1 + a / 256, a).astype(dt)  # for the product
+    starts = [(0, 0), (w - 1, h - 1), (w // 2, h // 3), (bs[0], bs[1] - 1), (w - 1, 0), (3, h - 2)]
+    for red, gf, of, x in [("sum", G.rsum, O.rsum, a), ("max", G.rmax, O.rmax, a), ("min", G.rmin, O.rmin, a),
+                           ("prod", G.rprod, O.rprod, p)]:
+        A, oa = G.from_array(x, bs), O.from_array(x, bs)
+        for s in starts:
+            print("  %s from %s" % (red, s), flush=True)
+            assert_exact(G.to_array(gf(A, list(s))), of(oa, s).val, "r%s from %s" % (red, s))
+
+
+def test_viewshed_matches_oracle(G, O):
+    """python/view.py's viewshed (normalised DEM, slope to the observer, rmax, visibility)
+    against the oracle; every step is one IEEE operation, so bit-exact."""
+    print("case viewshed 520x390 f32, blocks 128x128", flush=True)
+    w, h, bs, obs, oh = 520, 390, (128, 128), (200, 150), 0.5
+    dem = dem_array(w, h, 5)
+    yy, xx = np.mgrid[0:h, 0:w]
+    dx, dy = (obs[0] - xx).astype(np.int32), (obs[1] - yy).astype(np.int32)
+    dist = np.sqrt((dx * dx + dy * dy).astype(np.float32))  # view.py distance(): sqrt(x*x + y*y + 0.0)
+
+    def viewshed(M, D, DIST):
+        norm = (D - M.zmin(D)) / (M.zmax(D) - M.zmin(D))
+        slope = (norm - oh) / DIST
+        return (M.rmax(slope, list(obs)) * DIST + oh) - norm
+
+    got = G.to_array(viewshed(G, G.from_array(dem, bs), G.from_array(dist, bs)))
+    ref = viewshed(O, O.from_array(dem, bs), O.from_array(dist, bs)).val
+    assert_exact(got, ref, "viewshed")

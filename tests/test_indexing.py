@@ -350,3 +350,71 @@ def test_stream_reads_in_bounds(ds, bs, h0, h1, tile, size):
                         cbe1 = min(bs[1], ds[1] - (cb // nb[0]) * bs[1])
                         assert 0 <= cx < be0 and 0 <= cy < cbe1, (b, x, y, cx, cy)
                         assert 0 <= (cb * bs[1] + cy) * bs[0] + cx < total
+
+
+# ---- radial scan (map_amd/csrc/radial.hip) ---------------------------------------------
+
+RADIAL_DIRS = [(0, -1, 1, 0), (1, 0, 0, -1), (1, 0, 0, 1), (0, 1, 1, 0),
+               (0, 1, -1, 0), (-1, 0, 0, 1), (-1, 0, 0, -1), (0, -1, -1, 0)]
+
+
+def _radial_launches(ds, s, K, NT=256):
+    """The host loop of radial::run: per band, the octants' work-group counts."""
+    HS = NT - K
+    octs = []
+    for fx, fy, sx, sy in RADIAL_DIRS:
+        L0 = (ds[0] - 1 - s[0] if fx > 0 else s[0]) if fx else (ds[1] - 1 - s[1] if fy > 0 else s[1])
+        L1 = (ds[0] - 1 - s[0] if sx > 0 else s[0]) if sx else (ds[1] - 1 - s[1] if sy > 0 else s[1])
+        octs.append((fx, fy, sx, sy, L0, L1))
+    maxL0 = max(o[4] for o in octs)
+    for c0 in range(0, maxL0 + 1, K):
+        wgs = []
+        for o, (fx, fy, sx, sy, L0, L1) in enumerate(octs):
+            if c0 > L0:
+                continue
+            rmax = min(c0 + K - 1, L1)
+            for j in range((rmax + 1 + HS - 1) // HS):
+                wgs.append((o, j))
+        if wgs:
+            yield c0, octs, wgs
+
+
+@pytest.mark.parametrize("ds,bs,s,K", [((300, 220), (128, 64), (150, 73), 64), ((700, 90), (256, 32), (0, 0), 64),
+                                      ((257, 129), (64, 64), (256, 128), 32), ((33, 17), (16, 16), (16, 0), 64),
+                                      ((520, 390), (128, 128), (200, 150), 64), ((5, 300), (4, 64), (2, 299), 32)])
+def test_radial_reads_and_writes_in_bounds(ds, bs, s, K):
+    """Emulates every global access of k_radial: loads of the band's inputs, the read of the
+    previous column, the owners' stores; all land in the allocation at the intended cell, the
+    previous column was stored by an earlier band, and the stores cover every cell."""
+    NT = 256
+    HS = NT - K
+    nb = ((ds[0] + bs[0] - 1) // bs[0], (ds[1] + bs[1] - 1) // bs[1])
+    total = nb[0] * nb[1] * bs[0] * bs[1]
+
+    def off(x, y):
+        assert 0 <= x < ds[0] and 0 <= y < ds[1], (x, y)
+        bx, by = x // bs[0], y // bs[1]
+        o = ((by * nb[0] + bx) * bs[1] + y % bs[1]) * bs[0] + x % bs[0]
+        assert 0 <= o < total
+        return o
+
+    stored = set()
+    for c0, octs, wgs in _radial_launches(ds, s, K, NT):
+        band = set()
+        for o, j in wgs:
+            fx, fy, sx, sy, L0, L1 = octs[o]
+            cend = min(c0 + K, L0 + 1)
+            for t in range(NT):
+                r = j * HS - K + t
+                if not (0 <= r <= L1):
+                    continue
+                if c0 > 0 and r <= c0 - 1:  # previous column, stored by an earlier band
+                    assert off(s[0] + (c0 - 1) * fx + r * sx, s[1] + (c0 - 1) * fy + r * sy) in stored
+                for c in range(c0, cend):
+                    if r <= c:
+                        a = off(s[0] + c * fx + r * sx, s[1] + c * fy + r * sy)
+                        if t >= K:
+                            band.add(a)
+        stored |= band
+    cells = {off(x, y) for y in range(ds[1]) for x in range(ds[0])}
+    assert stored == cells
