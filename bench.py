@@ -333,6 +333,50 @@ def serve_bench(M, block=1024, seconds=5.0, seed=9):
     return out
 
 
+def radial_bench(M, side=16384, block=1024, reps=3):
+    """Radial scan (rmax from the centre) of a resident side^2 FP32 DEM, and the viewshed
+    script of python/view.py on it. Device time of each evaluation from HIP events;
+    algorithmic traffic of the scan 8 B/cell (input read + output write)."""
+    import numpy as np
+    from map_amd import workloads as W
+    rng = np.random.Generator(np.random.PCG64(2))
+    dem = M.gen_dem((side, side), (block, block), W.dem_gaussians(rng, side, side, ref_side=4096.0 * side / 4096), 2)
+    M.eval(dem)
+    obs = [side // 2, side // 2]
+    out = {}
+    scan = M.rmax(dem, obs)
+    M.eval(scan)
+    devs, launches = [], None
+    for _ in range(reps):
+        M.release(scan)
+        l0 = M.stats()["launches"]
+        M.eval(scan)
+        devs.append(M.last_eval_ms())
+        launches = M.stats()["launches"] - l0
+    ms = min(devs)
+    gbs = 8.0 * side * side / (ms * 1e-3) / 1e9
+    out["rmax"] = {"device_ms": round(ms, 3), "gcells_s": round(side * side / (ms * 1e-3) / 1e9, 2), "launches": launches,
+                   "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    del scan
+    # python/view.py: normalise, slope to the observer, rmax, visibility (the distance raster from index())
+    h = 0.5
+    dx = obs[0] - M.index(dem, M.D1)
+    dy = obs[1] - M.index(dem, M.D2)
+    dist = M.sqrt(dx * dx + dy * dy + 0.0)
+    norm = (dem - M.zmin(dem)) / (M.zmax(dem) - M.zmin(dem))
+    view = (M.rmax((norm - h) / dist, obs) * dist + h) - norm
+    M.eval(view)
+    devs = []
+    for _ in range(reps):
+        M.release(view)
+        M.eval(view)
+        devs.append(M.last_eval_ms())
+    out["viewshed"] = {"device_ms": round(min(devs), 3), "gcells_s": round(side * side / (min(devs) * 1e-3) / 1e9, 2)}
+    del view, dist, norm, dx, dy
+    M.release(dem)
+    return {"workload": "radial scan from the centre of a %dx%d f32 DEM, blocks %dx%d" % (side, side, block, block), **out}
+
+
 def hillshade_bench(M, side=4096, block=512, reps=5):
     """The plain hillshade script (python/hill.py) on a side^2 FP32 DEM in
     block^2 blocks: one fused focal+local group, plain and batched forms."""
@@ -376,7 +420,7 @@ def main():
     ap.add_argument("--tile", default="", help="TX,TY,VX[,PF] tile shape of the Life kernel (implies --tune 0)")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the Life headline (skip hillshade, hillshade+zonal, batched tiles, serving)")
-    ap.add_argument("--only", choices=["hillshade", "hillshade_zonal", "batched_tiles", "serve"],
+    ap.add_argument("--only", choices=["hillshade", "hillshade_zonal", "batched_tiles", "serve", "radial"],
                     help="run one extra section alone (no Life headline), print its JSON")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a U8 copy of the input raster (a local cast kernel), for counter calibration")
@@ -389,7 +433,7 @@ def main():
     M.setupDevices("", M.DEV_GPU, "")
     if args.only:
         fn = {"hillshade": hillshade_bench, "hillshade_zonal": hillshade_zonal_bench,
-              "batched_tiles": batched_tiles_bench, "serve": serve_bench}[args.only]
+              "batched_tiles": batched_tiles_bench, "serve": serve_bench, "radial": radial_bench}[args.only]
         print(json.dumps({args.only: fn(M)}), flush=True)
         return
     bs = (args.block, args.block)
@@ -518,7 +562,7 @@ def main():
         M.set_option("graph", 0)
         extras = {}
         for name, fn in (("hillshade", hillshade_bench), ("hillshade_zonal", hillshade_zonal_bench),
-                         ("batched_tiles", batched_tiles_bench), ("serve", serve_bench)):
+                         ("batched_tiles", batched_tiles_bench), ("serve", serve_bench), ("radial", radial_bench)):
             t0 = time.perf_counter()
             try:
                 extras[name] = fn(M)

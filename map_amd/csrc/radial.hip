@@ -14,10 +14,13 @@
 // every octant by one band of K columns. A work-group owns a strip of HS rows
 // (one row per thread) and also runs the K rows below it (a trapezoid: row r
 // at column c needs rows r and r-1 at column c-1, so after K columns only the
-// HS owned rows are exact); strips never wait on each other. All input loads
-// of the band are issued first; each column exchanges the previous column's
-// values through LDS (one barrier). Results go to `out` (block-tiled, like
-// every raster); the next band reads its starting column back from there.
+// HS owned rows are exact); strips never wait on each other. Everything off
+// the recurrence comes first (the band's input loads, all in flight together,
+// and the interpolation weights); each column then exchanges the previous
+// column's values through LDS (one barrier) with no global memory operation
+// in the chain (a store there would make the next barrier wait for it); the
+// results are stored after the band to `out` (block-tiled, like every raster),
+// from which the next band reads its starting column.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -84,13 +87,6 @@ template <int OP> __device__ __forceinline__ double combine(double b, double x) 
   return fmin(b, x);
 }
 
-// The ray's value where it crosses column d0-1 (linear_interpolation in
-// runtime/skeleton/util.cpp:564; the float literals there are exact here).
-template <class T> __device__ __forceinline__ T crossing(T a, T b, int d0, int d1) {
-  const T w = (T)d1 / (T)d0 * (T)(d0 - 1) - (T)(d1 - 1);
-  return a * w + b * ((T)1 - w);
-}
-
 template <class T, int OP, int K>
 __global__ void __launch_bounds__(NT) k_radial(const T* __restrict__ in, T* __restrict__ out, Params P) {
   constexpr int HS = NT - K;  // rows a work-group owns
@@ -106,27 +102,34 @@ __global__ void __launch_bounds__(NT) k_radial(const T* __restrict__ in, T* __re
   const int cend = min(c0 + K, O.L0 + 1);        // columns [c0, cend)
   const bool row_ok = r >= 0 && r <= O.L1;
   const bool owner = t >= K;
-
-  // inputs of the band, loaded up front
-  T xin[K];
-  Pos pos;
-  if (row_ok) pos.init(P, P.s0 + c0 * O.fx + r * O.sx, P.s1 + c0 * O.fy + r * O.sy);
-#pragma unroll
-  for (int k = 0; k < K; k++) {
-    const int c = c0 + k;
-    xin[k] = (T)0;
-    if (row_ok && c < cend && r <= c) xin[k] = in[pos.off(P)];
-    if (row_ok && c + 1 < cend) pos.step(P, O.fx, O.fy);
-  }
+  // Row clamped into the octant's rows, so every address below is a raster
+  // cell: all loads are issued unconditionally (no branch between them), so
+  // they are in flight together; values of cells outside the octant are unused.
+  const int rc = r < 0 ? 0 : (r > O.L1 ? O.L1 : r);
+  const int x0 = P.s0 + c0 * O.fx + rc * O.sx, y0 = P.s1 + c0 * O.fy + rc * O.sy;  // cell (c0, rc)
 
   // the previous column's value of this row
   T prev = (T)0;
-  if (row_ok && c0 > 0 && r <= c0 - 1) {
+  if (c0 > 0) {  // uniform
     Pos q;
-    q.init(P, P.s0 + (c0 - 1) * O.fx + r * O.sx, P.s1 + (c0 - 1) * O.fy + r * O.sy);
+    q.init(P, x0 - O.fx, y0 - O.fy);
     prev = out[q.off(P)];
   }
-  if (row_ok) pos.init(P, P.s0 + c0 * O.fx + r * O.sx, P.s1 + c0 * O.fy + r * O.sy);
+  // the band's inputs and each column's interpolation weights (w, 1 - w), all
+  // before the per-column chain
+  T val[K], w[K], v1[K];
+  Pos pos;
+  pos.init(P, x0, y0);
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int c = c0 + k;
+    val[k] = in[pos.off(P)];
+    if (c + 1 < cend) pos.step(P, O.fx, O.fy);  // uniform: the last column repeats
+    w[k] = (T)0;
+    // linear_interpolation's weight (runtime/skeleton/util.cpp:564; its float literals are exact here)
+    if (r > 0 && r < c) w[k] = (T)r / (T)c * (T)(c - 1) - (T)(r - 1);
+    v1[k] = (T)1 - w[k];
+  }
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const int c = c0 + k;
@@ -135,17 +138,25 @@ __global__ void __launch_bounds__(NT) k_radial(const T* __restrict__ in, T* __re
     __syncthreads();
     const T nb = t > 0 ? xch[k & 1][t - 1] : prev;  // row r-1 at column c-1 (the halo's first row is never owned)
     if (row_ok && r <= c) {
-      T base;
-      if (c == 0) base = neutral<T, OP>();
-      else if (r == 0) base = prev;
-      else if (r == c) base = nb;
-      else base = crossing<T>(prev, nb, c, r);
-      const T x = xin[k];
-      const T v = __builtin_isfinite(x) ? combine<OP>(base, x) : base;
-      if (owner) out[pos.off(P)] = v;
-      prev = v;
+      const T mix = prev * w[k] + nb * v1[k];  // a*w + b*(1-w)
+      T base = r == c ? nb : mix;
+      base = r == 0 ? prev : base;
+      base = c == 0 ? neutral<T, OP>() : base;
+      const T x = val[k];
+      prev = __builtin_isfinite(x) ? combine<OP>(base, x) : base;
+      val[k] = prev;
     }
-    if (row_ok && c + 1 < cend) pos.step(P, O.fx, O.fy);
+  }
+  // the owned rows' results of the band
+  if (row_ok && owner) {
+    pos.init(P, x0, y0);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int c = c0 + k;
+      if (c >= cend) break;
+      if (r <= c) out[pos.off(P)] = val[k];
+      if (c + 1 < cend) pos.step(P, O.fx, O.fy);
+    }
   }
 }
 
