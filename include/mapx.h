@@ -59,6 +59,8 @@ typedef int ma_devicetype_t;
 typedef int ma_unary_t;
 typedef int ma_binary_t;
 typedef int ma_reduction_t;
+typedef int ma_percent_t;    /* PercentEnum (util/PercentType.hpp:18): AGE = 1, ILE = 2 */
+typedef int ma_diversity_t;  /* DiversityEnum (util/DiversityType.hpp:19-21): VARI = 1, MAJO, MINO, MEAN */
 
 /* ---- devices and evaluation ------------------------------------------------ */
 
@@ -104,6 +106,14 @@ ma_node_t *ma_neighbor(ma_node_t *node, ma_dim2_t coord);
 ma_node_t *ma_convolution(ma_node_t *node, ma_variant_t *mask_v, int mask_s, ma_dim2_t mask_ds);
 /* front/bindings.hpp:73 ma_focalFunc */
 ma_node_t *ma_focalFunc(ma_node_t *node, ma_variant_t *mask_v, int mask_s, ma_dim2_t mask_ds, ma_reduction_t type);
+/* front/bindings.hpp:73 ma_focalPercent (front/bindings.cpp:203-207): per cell, the window's count of
+ * cells equal to the centre (AGE) or below it (ILE), in the input's type, divided by the reference's
+ * neighbourhood size (2*H0+1)*H1*2+1 (the mask's values are not used, only its size). */
+ma_node_t *ma_focalPercent(ma_node_t *node, ma_variant_t *mask_v, int mask_s, ma_dim2_t mask_ds, ma_percent_t type);
+/* front/bindings.hpp:63 ma_diversity (front/bindings.cpp:145-151): per cell over 2..16 rasters of one
+ * shape and type: distinct values (VARI, S8), most / least frequent value (MAJO / MINO, first on
+ * ties), or MEAN (sum of values over the number of distinct values). */
+ma_node_t *ma_diversity(ma_node_t **vec, int size, ma_diversity_t type);
 /* front/bindings.hpp:76 ma_radialScan (front/bindings.cpp:218-221): radial scan from the cell
  * coord = {x, y}: each cell gets type (SUM/PROD/MAX/MIN) of its input and the value its ray
  * from coord carries in (interpolated between the two cells of the previous column); F32/F64 only. */

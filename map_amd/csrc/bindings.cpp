@@ -149,6 +149,18 @@ ma_node_t* ma_convolution(ma_node_t* node, ma_variant_t* mask_v, int mask_s, ma_
 ma_node_t* ma_focalFunc(ma_node_t* node, ma_variant_t* mask_v, int mask_s, ma_dim2_t mask_ds, ma_reduction_t type) {
   return guard([&] { return R().dag.focalFunc(need(node), make_mask(mask_v, mask_s, mask_ds), type); }, (Node*)nullptr);
 }
+ma_node_t* ma_focalPercent(ma_node_t* node, ma_variant_t* mask_v, int mask_s, ma_dim2_t mask_ds, ma_percent_t type) {
+  return guard([&] { return R().dag.focalPercent(need(node), make_mask(mask_v, mask_s, mask_ds), type); },
+               (Node*)nullptr);
+}
+ma_node_t* ma_diversity(ma_node_t** vec, int size, ma_diversity_t type) {
+  return guard([&] {
+    if (!vec || size < 0) throw std::runtime_error("diversity: bad node list");
+    std::vector<Node*> ps;
+    for (int i = 0; i < size; i++) ps.push_back(need(vec[i]));
+    return R().dag.diversity(ps, type);
+  }, (Node*)nullptr);
+}
 ma_node_t* ma_radialScan(ma_node_t* node, ma_reduction_t type, ma_dim2_t coord) {
   return guard([&] { return R().dag.radialScan(need(node), type, D(coord)); }, (Node*)nullptr);
 }

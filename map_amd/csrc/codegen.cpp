@@ -417,6 +417,36 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
           }
         break;
       }
+      case Kind::FocalPercent: {
+        // FocalSkeleton.cpp:375-403: count over the node's window (y outer, x inner) of
+        // centre == cell (AGE) or centre > cell (ILE), in the node's type, divided by the
+        // reference's nbh_size (skeleton/util.cpp:321-327), "(H0*2+1)*H1*2+1" over the
+        // kernel's halo constants H0, H1 (the group's halo)
+        const std::string c = "W.w" + LN(n->prev[0]) + "[H1][PAD + v]";
+        o << "    " << T << " " << V(n) << " = 0;\n";
+        const int h0 = n->halo(0), h1 = n->halo(1);
+        for (int i1 = -h1; i1 <= h1; i1++)
+          for (int i0 = -h0; i0 <= h0; i0++) {
+            std::string sv = "W.w" + LN(n->prev[0]) + "[H1 + (" + std::to_string(i1) + ")][PAD + v + (" +
+                             std::to_string(i0) + ")]";
+            o << "    " << V(n) << " = (" << T << ")(" << V(n) << " + (" << c << (n->op == 1 ? " == " : " > ") << sv << "));\n";
+          }
+        o << "    " << V(n) << " = (" << T << ")(" << V(n) << " / ((H0 * 2 + 1) * H1 * 2 + 1));\n";
+        break;
+      }
+      case Kind::Diversity: {
+        // Skeleton.cpp:259-288 with the helpers of skeleton/util.cpp:439-481
+        const std::string Ti = CT(n->prev[0]->meta.dt), L = LN(n), N = std::to_string(n->prev.size());
+        // the lists start zeroed: the slots past dn are never read, and left undefined they let
+        // the optimiser fold the unrolled loops around indeterminate values
+        o << "    " << Ti << " de" << L << "[" << N << "] = {}; int dc" << L << "[" << N << "] = {}; int dn" << L << " = 0;\n";
+        for (Node* p : n->prev)
+          o << "    mx::div_add<" << Ti << ", " << N << ">(" << V(p) << ", de" << L << ", dc" << L << ", dn" << L << ");\n";
+        const char* f = n->op == 1 ? nullptr : n->op == 2 ? "mx::div_major" : n->op == 3 ? "mx::div_minor" : "mx::div_mean";
+        if (!f) o << "    const " << T << " " << V(n) << " = (" << T << ")dn" << L << ";\n";
+        else o << "    const " << T << " " << V(n) << " = " << f << "<" << Ti << ", " << N << ">(de" << L << ", dc" << L << ", dn" << L << ");\n";
+        break;
+      }
       case Kind::ZonalReduc: case Kind::ZonalStats: break;  // accumulated below
       default: throw std::runtime_error(std::string("codegen: unexpected node kind ") + kind_name(n->kind));
     }

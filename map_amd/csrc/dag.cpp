@@ -16,6 +16,7 @@ const char* kind_name(Kind k) {
     case Kind::FocalFunc: return "FocalFunc"; case Kind::ZonalReduc: return "ZonalReduc";
     case Kind::ZonalStats: return "ZonalStats"; case Kind::Read: return "Read"; case Kind::Write: return "Write";
     case Kind::Scalar: return "Scalar"; case Kind::RadialScan: return "RadialScan";
+    case Kind::FocalPercent: return "FocalPercent"; case Kind::Diversity: return "Diversity";
   }
   return "?";
 }
@@ -87,8 +88,8 @@ Variant reduction_neutral(int red, int dt) {
 int Node::pattern() const {
   if (cut) return FREE;
   switch (kind) {
-    case Kind::Cast: case Kind::Unary: case Kind::Binary: case Kind::Conditional: return LOCAL;
-    case Kind::Neighbor: case Kind::Convolution: case Kind::FocalFunc: return FOCAL;
+    case Kind::Cast: case Kind::Unary: case Kind::Binary: case Kind::Conditional: case Kind::Diversity: return LOCAL;
+    case Kind::Neighbor: case Kind::Convolution: case Kind::FocalFunc: case Kind::FocalPercent: return FOCAL;
     case Kind::ZonalReduc: case Kind::ZonalStats: return ZONAL;
     case Kind::RadialScan: return RADIAL;
     default: return FREE;  // Constant, Index, Read, Write, Scalar (runtime/dag/Node.cpp:98)
@@ -98,7 +99,7 @@ int Node::pattern() const {
 int Node::halo(int i) const {
   switch (kind) {
     case Kind::Neighbor: return std::abs(coord.arr[i]);       // Neighbor.cpp:86
-    case Kind::Convolution: case Kind::FocalFunc: return mask.halo(i);  // Convolution.cpp:84
+    case Kind::Convolution: case Kind::FocalFunc: case Kind::FocalPercent: return mask.halo(i);  // Convolution.cpp:84
     default: return 0;
   }
 }
@@ -117,6 +118,8 @@ std::string Node::signature(bool float_weights) const {
         for (auto& w : mask.v) s << "," << variant_bits(w);
       break;
     case Kind::ZonalStats: s << "z" << nzones; break;
+    case Kind::FocalPercent: s << "p" << op << ":" << mask.ds.arr[0] << "x" << mask.ds.arr[1]; break;
+    case Kind::Diversity: s << "d" << op << ":" << prev.size() << name_dt(prev.empty() ? 0 : prev[0]->meta.dt); break;
     case Kind::RadialScan: s << "r" << op << "@" << coord.arr[0] << "," << coord.arr[1]; break;
     default: break;
   }
@@ -325,6 +328,44 @@ Node* Dag::zonalReduc(Node* p, int red) {
   n->op = red;
   n->prev = {p};
   n->value = reduction_neutral(red, p->meta.dt);
+  return add(std::move(n));
+}
+
+// FocalPercent (runtime/dag/FocalPercent.cpp:29-41): the share of the mask's
+// window (its values are not used, only its size) that equals the centre
+// (AGE) or that the centre exceeds (ILE); the input's type.
+Node* Dag::focalPercent(Node* p, const Mask& m, int type) {
+  if (!p || p->meta.nd != D2) fail("focalPercent: needs a D2 raster");
+  if (type != 1 && type != 2) fail("focalPercent: type must be AGE (1) or ILE (2)");
+  check_mask(m, p);
+  auto n = std::make_unique<Node>();
+  n->kind = Kind::FocalPercent;
+  n->meta = p->meta;
+  n->mask = m;
+  n->op = type;
+  n->prev = {p};
+  return add(std::move(n));
+}
+
+// Diversity (runtime/dag/Diversity.cpp:40-63) of two or more rasters of one
+// shape and type: per cell, the number of distinct values (VARI, an S8
+// raster), the most (MAJO) or least (MINO) frequent value, or MEAN; the
+// inputs' type otherwise.
+Node* Dag::diversity(const std::vector<Node*>& ps, int type) {
+  if (ps.size() < 2) fail("diversity: needs at least two rasters");
+  if (ps.size() > 16) fail("diversity: at most 16 rasters");
+  if (type < 1 || type > 4) fail("diversity: type must be VARI, MAJO, MINO or MEAN (1-4)");
+  for (Node* p : ps) {
+    if (!p || p->meta.nd != D2) fail("diversity: needs D2 rasters");
+    if (!dim2_eq(p->meta.ds, ps[0]->meta.ds) || !dim2_eq(p->meta.bs, ps[0]->meta.bs) || p->meta.dt != ps[0]->meta.dt)
+      fail("diversity: rasters differ in shape or type");
+  }
+  auto n = std::make_unique<Node>();
+  n->kind = Kind::Diversity;
+  n->meta = ps[0]->meta;
+  if (type == 1) n->meta.dt = S8;  // variety is always an integer (Diversity.cpp:56-58)
+  n->op = type;
+  n->prev = ps;
   return add(std::move(n));
 }
 

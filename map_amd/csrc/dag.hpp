@@ -19,7 +19,7 @@ enum class Kind : int {
   Neighbor, Convolution, FocalFunc,
   ZonalReduc, ZonalStats,
   Read, Write, Scalar,
-  RadialScan
+  RadialScan, FocalPercent, Diversity
 };
 const char* kind_name(Kind k);
 
@@ -105,6 +105,8 @@ class Dag {
   Node* zonalReduc(Node* p, int red);
   Node* zonalStats(Node* data, Node* zone, int nzones);
   Node* radialScan(Node* p, int red, Dim2 start);
+  Node* focalPercent(Node* p, const Mask& m, int type);
+  Node* diversity(const std::vector<Node*>& ps, int type);
   Node* read(const std::string& path);
   Node* input(std::shared_ptr<HostRaster> r);
   Node* write(Node* p, const std::string& path);

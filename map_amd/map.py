@@ -455,6 +455,25 @@ def _focal(raster, lst, red):
     return Raster(_lib.ma_focalFunc(raster, ptr, n, ds, red))
 
 
+AGE, ILE = 1, 2                      # PercentEnum (util/PercentType.hpp:18)
+VARI, MAJO, MINO, MEAN = 1, 2, 3, 4  # DiversityEnum (util/DiversityType.hpp:19-21)
+
+
+def focal_percent(raster, lst, kind):
+    """Share of the window (its size from the mask lst) equal to the centre (AGE) or below
+    it (ILE); the reference's ma_focalPercent (not exposed by its map.py)."""
+    ptr, n, ds, _keep = _mask(lst)
+    return Raster(_lib.ma_focalPercent(raster, ptr, n, ds, kind))
+
+
+def diversity(rasters, kind):
+    """Per-cell VARI / MAJO / MINO / MEAN over rasters of one shape and type; the reference's
+    ma_diversity (not exposed by its map.py)."""
+    rs = list(rasters)
+    arr = (Node * len(rs))(*[r._node for r in rs])
+    return Raster(_lib.ma_diversity(ct.cast(arr, ct.POINTER(Node)), len(rs), kind))
+
+
 def fmin(raster, lst):
     return _focal(raster, lst, MIN)
 
@@ -803,6 +822,8 @@ _proto('ma_convolution', [Raster, ct.POINTER(Variant), ct.c_int, Array], Node)
 _proto('ma_focalFunc', [Raster, ct.POINTER(Variant), ct.c_int, Array, ct.c_int], Node)
 _proto('ma_zonalReduc', [Raster, ct.c_int], Node)
 _proto('ma_radialScan', [Raster, ct.c_int, Array], Node)
+_proto('ma_focalPercent', [Raster, ct.POINTER(Variant), ct.c_int, Array, ct.c_int], Node)
+_proto('ma_diversity', [ct.POINTER(Node), ct.c_int, ct.c_int], Node)
 _proto('ma_zonalStats', [Raster, Raster, ct.c_int], Node)
 _proto('ma_zonalTable', [Raster, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double), ct.POINTER(ct.c_double),
                          ct.POINTER(ct.c_longlong), ct.c_int], ct.c_int)

@@ -413,6 +413,88 @@ def _focal(x, lst, red):
     return R(res.reshape(ds[1], ds[0]).astype(src.dtype), x.dt, x.bs)
 
 
+AGE, ILE = 1, 2
+VARI, MAJO, MINO, MEAN = 1, 2, 3, 4
+
+
+def focal_percent(x, lst, kind):
+    """FocalPercent (runtime/skeleton/FocalSkeleton.cpp:375-403): count, over the window of the
+    mask's size (y outer, x inner, border rule), of centre == cell (AGE) or centre > cell (ILE),
+    accumulated in the raster's type, then divided by the reference's nbh_size
+    (skeleton/util.cpp:321-327): "(H0*2+1)*H1*2+1", i.e. (2*h0+1)*h1*2+1 for a node alone
+    (C precedence; 7 for a 3x3 window)."""
+    ncol, nrow = len(lst[0]), len(lst)
+    h0, h1 = ncol // 2, nrow // 2
+    ds = x.datasize()
+    idx, nt = taps(ds, x.bs, h0, h1)
+    src = np.asarray(x.val).reshape(-1)
+    g = src[idx.reshape(-1, nt)]
+    with np.errstate(invalid="ignore"):
+        hit = (g == src[:, None]) if kind == AGE else (src[:, None] > g)
+    cnt = hit.sum(axis=1)  # at most 2*h0+1 by 2*h1+1: exact in every type
+    nbh = (2 * h0 + 1) * h1 * 2 + 1
+    if x.dt in FLOATS:
+        T = src.dtype.type
+        res = (cnt.astype(T) / T(nbh)).astype(T)
+    else:
+        res = _conv(cnt // nbh, x.dt)  # non-negative: C division = floor
+    return R(np.asarray(res).reshape(ds[1], ds[0]).astype(src.dtype), x.dt, x.bs)
+
+
+def _c_int(v, dt):
+    """A Python integer wrapped to the C type dt."""
+    bits = np.dtype(NP[dt]).itemsize * 8
+    v &= (1 << bits) - 1
+    if dt in (S8, S16, S32, S64) and v >= 1 << (bits - 1):
+        v -= 1 << bits
+    return v
+
+
+def diversity(xs, kind):
+    """Diversity (runtime/skeleton/Skeleton.cpp:259-288, helpers skeleton/util.cpp:439-481) per
+    cell over the rasters in order: distinct values in order of first appearance (== equality)
+    with counts; VARI = their number (S8), MAJO / MINO = the first most / least frequent,
+    MEAN = (sum of value * count, accumulated in the type with C conversions) / number of
+    distinct values. Per-cell Python loop: small rasters only."""
+    vals = [np.asarray(x.val).reshape(-1) for x in xs]
+    dt = xs[0].dt
+    T = vals[0].dtype.type
+    n = vals[0].size
+    out = np.empty(n, np.int8 if kind == VARI else vals[0].dtype)
+    fl = dt in FLOATS
+    for i in range(n):
+        el, cnt = [], []
+        for v in vals:
+            e = v[i]
+            for k in range(len(el)):
+                if e == el[k]:
+                    cnt[k] += 1
+                    break
+            else:
+                el.append(e)
+                cnt.append(1)
+        if kind == VARI:
+            out[i] = len(el)
+        elif kind in (MAJO, MINO):
+            m = 0
+            for k in range(1, len(el)):
+                if (cnt[k] > cnt[m]) if kind == MAJO else (cnt[k] < cnt[m]):
+                    m = k
+            out[i] = el[m]
+        elif fl:
+            mean = T(0)
+            for e, c in zip(el, cnt):
+                mean = T(mean + T(e * T(c)))
+            out[i] = T(mean / T(len(el)))
+        else:
+            mean = 0
+            for e, c in zip(el, cnt):
+                mean = _c_int(mean + int(e) * c, dt)
+            q = abs(mean) // len(el)
+            out[i] = _c_int(q if mean >= 0 else -q, dt)
+    return R(out.reshape(np.asarray(xs[0].val).shape), S8 if kind == VARI else dt, xs[0].bs)
+
+
 def fmin(x, lst):
     return _focal(x, lst, MIN)
 

@@ -425,3 +425,38 @@ def test_viewshed_matches_oracle(G, O):
     got = G.to_array(viewshed(G, G.from_array(dem, bs), G.from_array(dist, bs)))
     ref = viewshed(O, O.from_array(dem, bs), O.from_array(dist, bs)).val
     assert_exact(got, ref, "viewshed")
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.int32, np.uint8, np.float64])
+def test_focal_percent_matches_oracle(G, O, dt):
+    """ma_focalPercent (AGE, ILE) over 3x3, 5x3 and 3x5 windows, block seams and edges, against
+    the oracle bit for bit (values drawn from a few levels so equal neighbours are common)."""
+    print("case focal percent", np.dtype(dt).name, flush=True)
+    w, h, bs = 130, 90, (64, 32)
+    a = np.random.default_rng(21).integers(0, 4, (h, w)).astype(dt)
+    A, oa = G.from_array(a, bs), O.from_array(a, bs)
+    for mask in ([[1, 1, 1]] * 3, [[1] * 5] * 3, [[1, 1, 1]] * 5):
+        for kind in (G.AGE, G.ILE):
+            assert_exact(G.to_array(G.focal_percent(A, mask, kind)), O.focal_percent(oa, mask, kind).val,
+                         "percent %d %dx%d" % (kind, len(mask[0]), len(mask)))
+
+
+@pytest.mark.parametrize("dt,n", [(np.int32, 4), (np.uint8, 5), (np.float32, 3), (np.int16, 2), (np.float64, 4)])
+def test_diversity_matches_oracle(G, O, dt, n):
+    """ma_diversity (VARI, MAJO, MINO, MEAN) over n rasters of a few levels (NaNs among the
+    floats) against the oracle bit for bit; U8 means wrap as in C."""
+    print("case diversity", np.dtype(dt).name, n, flush=True)
+    w, h, bs = 70, 45, (32, 32)
+    rng = np.random.default_rng(22 + n)
+    xs = []
+    for i in range(n):
+        a = rng.integers(0, 4, (h, w)).astype(dt) * (60 if dt == np.uint8 else 1)
+        if np.issubdtype(dt, np.floating):
+            a[rng.random((h, w)) < 0.05] = np.nan
+        xs.append(a)
+    G_in, O_in = [G.from_array(a, bs) for a in xs], [O.from_array(a, bs) for a in xs]
+    for kind in (G.VARI, G.MAJO, G.MINO, G.MEAN):
+        got = G.to_array(G.diversity(G_in, kind))
+        want = O.diversity(O_in, kind).val
+        assert got.dtype == want.dtype
+        assert_exact(got, want, "diversity %d" % kind)
