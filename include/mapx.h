@@ -120,6 +120,28 @@ ma_node_t *ma_diversity(ma_node_t **vec, int size, ma_diversity_t type);
 ma_node_t *ma_radialScan(ma_node_t *node, ma_reduction_t type, ma_dim2_t coord);
 /* front/bindings.hpp:75 ma_zonalReduc: whole-raster reduction to a D0 value. */
 ma_node_t *ma_zonalReduc(ma_node_t *node, ma_reduction_t type);
+/* front/bindings.hpp:70 ma_spreadNeighbor (front/bindings.cpp:177-180): per cell, the reduction
+ * (from its neutral value) of the 3x3 neighbours whose D8 code in `dir` (1 E, 2 SE, 4 S, 8 SW,
+ * 16 W, 32 NW, 64 N, 128 NE) points at the cell, y outer / x inner, with the focal border rule;
+ * the input's type. */
+ma_node_t *ma_spreadNeighbor(ma_node_t *node, ma_node_t *dir, ma_reduction_t type);
+/* front/bindings.hpp:77 ma_spreadScan (front/bindings.cpp:225-228): accumulation of `node` along
+ * the U8 D8 directions `dir` (SUM only): each cell gets the total that leaves it towards its
+ * downstream cell (its value plus all its upstream totals); cells with no downstream cell in
+ * the raster, on a direction cycle or downstream of one get 0. */
+ma_node_t *ma_spreadScan(ma_node_t *node, ma_node_t *dir, ma_reduction_t type);
+/* front/bindings.hpp:79 ma_stats (front/bindings.cpp:232-235): the input's values, with its
+ * maximum and minimum computed in the same evaluation (read them with ma_statsRange). */
+ma_node_t *ma_stats(ma_node_t *node);
+/* front/bindings.hpp:80 ma_barrier (front/bindings.cpp:237-240): the input's values; the
+ * groups before and after it are never fused. */
+ma_node_t *ma_barrier(ma_node_t *node);
+/* front/bindings.hpp:65 ma_access (front/bindings.cpp:155-158): the value of cell coord = {x, y}
+ * as a D0 node. */
+ma_node_t *ma_access(ma_node_t *node, ma_dim2_t coord);
+/* front/bindings.hpp:66 ma_lhsAccess (front/bindings.cpp:160-163): `lhs` with cell coord = {x, y}
+ * replaced by the D0 value `rhs` (of lhs's type). */
+ma_node_t *ma_lhsAccess(ma_node_t *lhs, ma_node_t *rhs, ma_dim2_t coord);
 
 /* ---- extensions (no reference counterpart) ---------------------------------- */
 
@@ -162,7 +184,9 @@ const char *ma_plan(ma_node_t **vec, int num);
 /* Compiles every kernel of the plan with hiprtc for gfx950 (no device needed). */
 int ma_compilePlan(ma_node_t **vec, int num);
 /* Counters and device-memory figures of the runtime, as JSON. */
-const char *ma_stats(void);
+const char *ma_runStats(void);
+/* Maximum and minimum of an evaluated Stats node (ma_stats), in its data type. */
+int ma_statsRange(ma_node_t *node, ma_variant_t *max, ma_variant_t *min);
 /* Synthetic DEM generated in HBM: ng Gaussians given as rows (cx, cy, sigma,
  * amplitude) plus value-noise fBm with the given seed; F32, block-tiled. */
 ma_node_t *ma_genDem(ma_dim2_t ds, ma_dim2_t bs, const double *gauss, int ng, unsigned long long noise_seed);

@@ -168,8 +168,38 @@ ma_node_t* ma_radialScan(ma_node_t* node, ma_reduction_t type, ma_dim2_t coord) 
 ma_node_t* ma_zonalReduc(ma_node_t* node, ma_reduction_t type) {
   return guard([&] { return R().dag.zonalReduc(need(node), type); }, (Node*)nullptr);
 }
+ma_node_t* ma_spreadNeighbor(ma_node_t* node, ma_node_t* dir, ma_reduction_t type) {
+  return guard([&] { return R().dag.spreadNeighbor(need(node), need(dir), type); }, (Node*)nullptr);
+}
+ma_node_t* ma_spreadScan(ma_node_t* node, ma_node_t* dir, ma_reduction_t type) {
+  return guard([&] { return R().dag.spreadScan(need(node), need(dir), type); }, (Node*)nullptr);
+}
+ma_node_t* ma_stats(ma_node_t* node) {
+  return guard([&] { return R().dag.stats(need(node)); }, (Node*)nullptr);
+}
+ma_node_t* ma_barrier(ma_node_t* node) {
+  return guard([&] { return R().dag.barrier(need(node)); }, (Node*)nullptr);
+}
+ma_node_t* ma_access(ma_node_t* node, ma_dim2_t coord) {
+  return guard([&] { return R().dag.access(need(node), D(coord)); }, (Node*)nullptr);
+}
+ma_node_t* ma_lhsAccess(ma_node_t* lhs, ma_node_t* rhs, ma_dim2_t coord) {
+  return guard([&] { return R().dag.lhsAccess(need(lhs), need(rhs), D(coord)); }, (Node*)nullptr);
+}
 
 // ---- extensions
+
+int ma_statsRange(ma_node_t* node, ma_variant_t* max, ma_variant_t* min) {
+  return guard([&] {
+    need(node);
+    if (node->kind != Kind::Stats || node->aux.size() != 2) throw std::runtime_error("statsRange: not a Stats node");
+    for (Node* a : node->aux)
+      if (!a->evaluated) throw std::runtime_error("statsRange: the Stats node has not been evaluated");
+    if (max) *max = MV(node->aux[0]->value);
+    if (min) *min = MV(node->aux[1]->value);
+    return 0;
+  }, -1);
+}
 
 ma_node_t* ma_zonalStats(ma_node_t* data, ma_node_t* zone, int nzones) {
   return guard([&] { return R().dag.zonalStats(need(data), need(zone), nzones); }, (Node*)nullptr);
@@ -344,7 +374,7 @@ int ma_compilePlan(ma_node_t** vec, int num) {
   }, -1);
 }
 
-const char* ma_stats(void) {
+const char* ma_runStats(void) {
   Runtime& r = R();
   std::ostringstream s;
   s << "{\"evals\": " << r.cnt.evals << ", \"groups\": " << r.cnt.groups << ", \"launches\": " << r.cnt.launches

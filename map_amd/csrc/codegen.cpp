@@ -118,6 +118,23 @@ TileShape default_tile(const Group& g) {
   return t;
 }
 
+// `acc = OP(acc, v)` in the node's C type: fmax/fmin for floating MAX/MIN (the
+// reference's max/min builtins, ReductionType.cpp:66-78), a comparison for
+// integers, the operator otherwise.
+static std::string reduce_stmt(int op, int dt, const std::string& acc, const std::string& v) {
+  const std::string T = CT(dt);
+  std::ostringstream o;
+  if (op == MAX || op == MIN) {
+    if (dt == F32) o << acc << " = " << (op == MAX ? "fmaxf(" : "fminf(") << acc << ", " << v << ");";
+    else if (dt == F64) o << acc << " = " << (op == MAX ? "fmax(" : "fmin(") << acc << ", " << v << ");";
+    else o << "{ const " << T << " t_ = " << v << "; " << acc << " = (t_ " << (op == MAX ? ">" : "<") << " " << acc << ") ? t_ : " << acc << "; }";
+  } else {
+    const char* c = op == SUM ? "+" : op == PROD ? "*" : op == rAND ? "&" : "|";
+    o << acc << " = (" << T << ")(" << acc << " " << c << " " << v << ");";
+  }
+  return o.str();
+}
+
 KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, TileShape tile) {
   KernelSpec ks;
   ks.tile = tile;
@@ -149,8 +166,12 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     if (n->pattern() == FOCAL) focal.push_back(n);
     if (n->kind == Kind::ZonalReduc || n->kind == Kind::ZonalStats) zonal.push_back(n);
   }
-  for (Node* f : focal)
-    if (!contains(staged, f->prev[0])) staged.push_back(f->prev[0]);
+  for (Node* f : focal) {
+    // the windowed sources of a focal node: its input (and the direction raster of a SpreadNeighbor)
+    const size_t ns = f->kind == Kind::SpreadNeighbor ? 2 : 1;
+    for (size_t k = 0; k < ns; k++)
+      if (!contains(staged, f->prev[k])) staged.push_back(f->prev[k]);
+  }
   std::sort(staged.begin(), staged.end(), [](Node* a, Node* b) { return a->id < b->id; });
   for (Node* s : staged)
     if (!member.count(s)) throw std::runtime_error("codegen: focal source outside its group");
@@ -405,18 +426,30 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
           for (int i0 = -h0; i0 <= h0; i0++) {
             std::string sv = "W.w" + LN(n->prev[0]) + "[H1 + (" + std::to_string(i1) + ")][PAD + v + (" +
                              std::to_string(i0) + ")]";
-            if (n->op == MAX || n->op == MIN) {
-              if (n->meta.dt == F32) o << "    " << V(n) << " = " << (n->op == MAX ? "fmaxf(" : "fminf(") << V(n) << ", " << sv << ");\n";
-              else if (n->meta.dt == F64) o << "    " << V(n) << " = " << (n->op == MAX ? "fmax(" : "fmin(") << V(n) << ", " << sv << ");\n";
-              else o << "    { const " << T << " t_ = " << sv << "; " << V(n) << " = (t_ " << (n->op == MAX ? ">" : "<") << " "
-                     << V(n) << ") ? t_ : " << V(n) << "; }\n";
-            } else {
-              const char* c = n->op == SUM ? "+" : n->op == PROD ? "*" : n->op == rAND ? "&" : "|";
-              o << "    " << V(n) << " = (" << T << ")(" << V(n) << " " << c << " " << sv << ");\n";
-            }
+            o << "    " << reduce_stmt(n->op, n->meta.dt, V(n), sv) << "\n";
           }
         break;
       }
+      case Kind::SpreadNeighbor: {
+        // the neighbours (y outer, x inner, centre skipped) whose D8 code points at
+        // the cell: code[i1 + 1][i0 + 1] = {{2,4,8},{1,0,16},{128,64,32}}, the
+        // reference's dircod (skeleton/util.cpp:628) and flowAccuGather (front/Raster.cpp:586-601)
+        static const int code[3][3] = {{2, 4, 8}, {1, 0, 16}, {128, 64, 32}};
+        o << "    " << T << " " << V(n) << " = " << lit(reduction_neutral(n->op, n->meta.dt)) << ";\n";
+        for (int i1 = -1; i1 <= 1; i1++)
+          for (int i0 = -1; i0 <= 1; i0++) {
+            if (i0 == 0 && i1 == 0) continue;
+            const std::string at = "[H1 + (" + std::to_string(i1) + ")][PAD + v + (" + std::to_string(i0) + ")]";
+            o << "    if (W.w" << LN(n->prev[1]) << at << " == " << code[i1 + 1][i0 + 1] << ") { "
+              << reduce_stmt(n->op, n->meta.dt, V(n), "W.w" + LN(n->prev[0]) + at) << " }\n";
+          }
+        break;
+      }
+      case Kind::Stats: case Kind::Barrier:  // the input's values (ZonalSkeleton.cpp:290-292, Skeleton.cpp:328-330)
+        o << "    const " << T << " " << V(n) << " = " << V(n->prev[0]) << ";\n"; break;
+      case Kind::LhsAccess:  // Skeleton.cpp:290-297: the one cell takes the right-hand value
+        o << "    const " << T << " " << V(n) << " = (k.bx * k.BS0 + x == " << n->coord.arr[0] << " && k.by * k.BS1 + y == "
+          << n->coord.arr[1] << ") ? (" << T << ")" << V(n->prev[1]) << " : " << V(n->prev[0]) << ";\n"; break;
       case Kind::FocalPercent: {
         // FocalSkeleton.cpp:375-403: count over the node's window (y outer, x inner) of
         // centre == cell (AGE) or centre > cell (ILE), in the node's type, divided by the

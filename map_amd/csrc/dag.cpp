@@ -17,6 +17,9 @@ const char* kind_name(Kind k) {
     case Kind::ZonalStats: return "ZonalStats"; case Kind::Read: return "Read"; case Kind::Write: return "Write";
     case Kind::Scalar: return "Scalar"; case Kind::RadialScan: return "RadialScan";
     case Kind::FocalPercent: return "FocalPercent"; case Kind::Diversity: return "Diversity";
+    case Kind::SpreadNeighbor: return "SpreadNeighbor"; case Kind::SpreadScan: return "SpreadScan";
+    case Kind::Stats: return "Stats"; case Kind::Barrier: return "Barrier";
+    case Kind::Access: return "Access"; case Kind::LhsAccess: return "LhsAccess";
   }
   return "?";
 }
@@ -88,10 +91,15 @@ Variant reduction_neutral(int red, int dt) {
 int Node::pattern() const {
   if (cut) return FREE;
   switch (kind) {
-    case Kind::Cast: case Kind::Unary: case Kind::Binary: case Kind::Conditional: case Kind::Diversity: return LOCAL;
-    case Kind::Neighbor: case Kind::Convolution: case Kind::FocalFunc: case Kind::FocalPercent: return FOCAL;
+    case Kind::Cast: case Kind::Unary: case Kind::Binary: case Kind::Conditional: case Kind::Diversity:
+    case Kind::Access: case Kind::LhsAccess: return LOCAL;  // Access.hpp:44, LhsAccess.hpp:45
+    case Kind::Neighbor: case Kind::Convolution: case Kind::FocalFunc: case Kind::FocalPercent:
+    case Kind::SpreadNeighbor: return FOCAL;  // SpreadNeighbor.hpp:42
     case Kind::ZonalReduc: case Kind::ZonalStats: return ZONAL;
     case Kind::RadialScan: return RADIAL;
+    case Kind::SpreadScan: return SPREAD;  // SpreadScan.hpp:48
+    case Kind::Stats: return LOCAL;        // an identity statement here (the reference: SPECIAL, Stats.hpp:43)
+    case Kind::Barrier: return BARRIER;    // Barrier.hpp:39
     default: return FREE;  // Constant, Index, Read, Write, Scalar (runtime/dag/Node.cpp:98)
   }
 }
@@ -100,6 +108,7 @@ int Node::halo(int i) const {
   switch (kind) {
     case Kind::Neighbor: return std::abs(coord.arr[i]);       // Neighbor.cpp:86
     case Kind::Convolution: case Kind::FocalFunc: case Kind::FocalPercent: return mask.halo(i);  // Convolution.cpp:84
+    case Kind::SpreadNeighbor: return 1;  // SpreadNeighbor.cpp:88-90
     default: return 0;
   }
 }
@@ -121,6 +130,9 @@ std::string Node::signature(bool float_weights) const {
     case Kind::FocalPercent: s << "p" << op << ":" << mask.ds.arr[0] << "x" << mask.ds.arr[1]; break;
     case Kind::Diversity: s << "d" << op << ":" << prev.size() << name_dt(prev.empty() ? 0 : prev[0]->meta.dt); break;
     case Kind::RadialScan: s << "r" << op << "@" << coord.arr[0] << "," << coord.arr[1]; break;
+    case Kind::SpreadNeighbor: s << "sn" << op << name_dt(prev[1]->meta.dt); break;
+    case Kind::SpreadScan: s << "ss" << op << name_dt(prev[1]->meta.dt); break;
+    case Kind::Access: case Kind::LhsAccess: s << "a" << coord.arr[0] << "," << coord.arr[1]; break;
     default: break;
   }
   return s.str();
@@ -152,6 +164,7 @@ Node* Dag::add(std::unique_ptr<Node> n) {
   else n->id = id_count_++;
   Node* raw = n.get();
   for (auto* p : raw->prev) { p->next.push_back(raw); p->ref++; }  // Node::addNext (Node.cpp:83-86)
+  for (auto* a : raw->aux) a->ref++;
   raw->gkey = k;
   nodes_.push_back(std::move(n));
   gvn_.emplace(std::move(k), raw);
@@ -369,6 +382,106 @@ Node* Dag::diversity(const std::vector<Node*>& ps, int type) {
   return add(std::move(n));
 }
 
+// SpreadNeighbor (runtime/dag/SpreadNeighbor.cpp:31-47): per cell, the
+// reduction of the neighbours' values (3x3 window, y outer, x inner, centre
+// excluded) whose D8 direction code in `dir` points at the cell, starting from
+// the reduction's neutral value; the input's type. It is the gather step of the
+// reference's flowAccuGather (front/Raster.cpp:583-606) as one focal node.
+Node* Dag::spreadNeighbor(Node* p, Node* dir, int red) {
+  if (!p || !dir || p->meta.nd != D2 || dir->meta.nd != D2) fail("spreadNeighbor: needs two D2 rasters");
+  if (!dim2_eq(p->meta.ds, dir->meta.ds) || !dim2_eq(p->meta.bs, dir->meta.bs))
+    fail("spreadNeighbor: the value and direction rasters differ in shape");
+  if (!reduction_valid(red)) fail("spreadNeighbor: bad reduction");
+  if (p->meta.bs.arr[0] < 1 || p->meta.bs.arr[1] < 1) fail("spreadNeighbor: bad block size");
+  auto n = std::make_unique<Node>();
+  n->kind = Kind::SpreadNeighbor;
+  n->meta = p->meta;
+  n->op = red;
+  n->prev = {p, dir};
+  return add(std::move(n));
+}
+
+// SpreadScan (runtime/dag/SpreadScan.cpp:31-45): accumulation of `p` along the
+// D8 flow directions of `dir` (U8). Each cell's result is the total that leaves
+// it towards its downstream cell: its own value plus everything that reaches it
+// from upstream; a cell without a downstream cell inside the raster keeps the
+// neutral value. SUM only: the reference's kernel applies `+=` whatever the
+// reduction (SpreadingSkeleton.cpp:318-320).
+Node* Dag::spreadScan(Node* p, Node* dir, int red) {
+  if (!p || !dir || p->meta.nd != D2 || dir->meta.nd != D2) fail("spreadScan: needs two D2 rasters");
+  if (!dim2_eq(p->meta.ds, dir->meta.ds) || !dim2_eq(p->meta.bs, dir->meta.bs))
+    fail("spreadScan: the value and direction rasters differ in shape");
+  if (dir->meta.dt != U8) fail("spreadScan: directions must be U8 (SpreadScan.cpp:37)");
+  if (red != SUM) fail("spreadScan: only SUM is supported (the reference accumulates with +=)");
+  if (p->meta.dt == B8) fail("spreadScan: needs a numeric raster");
+  auto n = std::make_unique<Node>();
+  n->kind = Kind::SpreadScan;
+  n->meta = p->meta;
+  n->op = red;
+  n->prev = {p, dir};
+  return add(std::move(n));
+}
+
+// Stats (runtime/dag/Stats.cpp:30-61): the input's values, with its global
+// maximum and minimum (ZonalReduc MAX / MIN of the input) computed whenever
+// the Stats node is evaluated: the node holds them (aux) as the reference's
+// prev_list does, without making them inputs of its own statement.
+Node* Dag::stats(Node* p) {
+  if (!p || p->meta.nd != D2) fail("stats: needs a D2 raster");
+  Node* mx = zonalReduc(p, MAX);
+  Node* mn = zonalReduc(p, MIN);
+  auto n = std::make_unique<Node>();
+  n->kind = Kind::Stats;
+  n->meta = p->meta;
+  n->prev = {p};
+  n->aux = {mx, mn};
+  return add(std::move(n));
+}
+
+// Barrier (runtime/dag/Barrier.cpp): the input's values; its BARRIER pattern
+// keeps the groups on either side apart (Pattern.cpp:65-119).
+Node* Dag::barrier(Node* p) {
+  if (!p) fail("barrier: null node");
+  auto n = std::make_unique<Node>();
+  n->kind = Kind::Barrier;
+  n->meta = p->meta;
+  n->prev = {p};
+  return add(std::move(n));
+}
+
+// Access (runtime/dag/Access.cpp:29-40): the value of one cell of a D2 raster,
+// a D0 value of the raster's type.
+Node* Dag::access(Node* p, Dim2 cell) {
+  if (!p || p->meta.nd != D2) fail("access: needs a D2 raster");
+  if (cell.n != 2 || cell.arr[0] < 0 || cell.arr[1] < 0 || cell.arr[0] >= p->meta.ds.arr[0] ||
+      cell.arr[1] >= p->meta.ds.arr[1])
+    fail("access: the cell must lie inside the raster");
+  auto n = std::make_unique<Node>();
+  n->kind = Kind::Access;
+  n->meta.dt = p->meta.dt;
+  n->meta.nd = D0;
+  n->meta.mo = p->meta.mo;
+  n->coord = cell;
+  n->prev = {p};
+  return add(std::move(n));
+}
+
+// LhsAccess (runtime/dag/LhsAccess.cpp; Skeleton.cpp:290-297): `lhs` with the
+// one cell replaced by the D0 value `rhs` (converted to lhs's type by the
+// caller, python/map.py:213).
+Node* Dag::lhsAccess(Node* lhs, Node* rhs, Dim2 cell) {
+  if (!lhs || !rhs || lhs->meta.nd != D2 || rhs->meta.nd != D0) fail("lhsAccess: needs a D2 raster and a D0 value");
+  if (cell.n != 2 || cell.arr[0] < 0 || cell.arr[1] < 0 || cell.arr[0] >= lhs->meta.ds.arr[0] ||
+      cell.arr[1] >= lhs->meta.ds.arr[1])
+    fail("lhsAccess: the cell must lie inside the raster");
+  auto n = std::make_unique<Node>();
+  n->kind = Kind::LhsAccess;
+  n->meta = lhs->meta;
+  n->coord = cell;
+  n->prev = {lhs, rhs};
+  return add(std::move(n));
+}
+
 // Radial scan from the start cell (RadialScan::Factory, runtime/dag/RadialScan.cpp:34-45): a
 // raster of the input's shape and type. Floating inputs only: the reference's
 // interpolation weight is meaningless for integers (RadiatingSkeleton's
@@ -447,6 +560,8 @@ void Dag::collect(const std::function<void(Node*)>& on_delete) {
       if (f != p->next.end()) { p->next.erase(f); p->ref--; }
     }
     n->prev.clear();
+    for (Node* a : n->aux) a->ref--;
+    n->aux.clear();
   }
   if (!any) return;
   std::vector<std::unique_ptr<Node>> keep;

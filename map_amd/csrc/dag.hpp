@@ -19,7 +19,8 @@ enum class Kind : int {
   Neighbor, Convolution, FocalFunc,
   ZonalReduc, ZonalStats,
   Read, Write, Scalar,
-  RadialScan, FocalPercent, Diversity
+  RadialScan, FocalPercent, Diversity,
+  SpreadNeighbor, SpreadScan, Stats, Barrier, Access, LhsAccess
 };
 const char* kind_name(Kind k);
 
@@ -49,6 +50,9 @@ struct Node {
   Kind kind;
   Meta meta;
   std::vector<Node*> prev, next;
+  // nodes evaluated together with this one and kept alive by it, without a data
+  // dependency (the MAX / MIN reductions of a Stats node)
+  std::vector<Node*> aux;
   int ref = 0;  // references held by the front end (ma_increaseRef)
   std::string gkey;  // value-numbering key (set when the node enters the DAG)
 
@@ -56,7 +60,7 @@ struct Node {
   int op = 0;            // Unary/Binary/Reduction enum, Index dim, Cast target
   Variant cnst{};        // Constant
   Mask mask;             // Convolution / FocalFunc
-  Dim2 coord = dim2_none();  // Neighbor offset; RadialScan start cell
+  Dim2 coord = dim2_none();  // Neighbor offset; RadialScan start cell; Access / LhsAccess cell
   std::string path;      // Read / Write
   int nzones = 0;        // ZonalStats
   std::shared_ptr<HostRaster> host;  // Read of an in-memory raster
@@ -107,6 +111,12 @@ class Dag {
   Node* radialScan(Node* p, int red, Dim2 start);
   Node* focalPercent(Node* p, const Mask& m, int type);
   Node* diversity(const std::vector<Node*>& ps, int type);
+  Node* spreadNeighbor(Node* p, Node* dir, int red);
+  Node* spreadScan(Node* p, Node* dir, int red);
+  Node* stats(Node* p);
+  Node* barrier(Node* p);
+  Node* access(Node* p, Dim2 cell);
+  Node* lhsAccess(Node* lhs, Node* rhs, Dim2 cell);
   Node* read(const std::string& path);
   Node* input(std::shared_ptr<HostRaster> r);
   Node* write(Node* p, const std::string& path);

@@ -132,7 +132,13 @@ int occupancy(const Function& f, int block) {
 
 void* malloc_device(size_t bytes) {
   void* p = nullptr;
-  check(hipMalloc(&p, bytes ? bytes : 1), "hipMalloc");
+  const hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    // a failed allocation leaves its error as the thread's last error; the caller may
+    // recover (free cached buffers and retry), so a later launch check must not see it
+    (void)hipGetLastError();
+    check(e, "hipMalloc");
+  }
   return p;
 }
 void free_device(void* p) { if (p) check(hipFree(p), "hipFree"); }

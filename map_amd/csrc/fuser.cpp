@@ -192,14 +192,20 @@ static bool same_geometry(const Group* a, const Group* b) {
   return dim2_eq(ma.ds, mb.ds) && dim2_eq(ma.bs, mb.bs);
 }
 
-// Radial groups run alone: their kernel (a sequential-along-the-ray recurrence,
-// device/radial.hip) has no fused per-cell statements. The reference fuses
-// LOCAL groups on either side of a RADIAL one (Pattern.cpp:71,95,149).
-static bool radial(const Group* g) { return pattern_is(g->pat, RADIAL); }
+// Groups that run alone: radial and spreading scans (precompiled kernels,
+// device/radial.hip and spread.hip, with no fused per-cell statements; the
+// reference fuses LOCAL groups on either side, Pattern.cpp:71,95,149) and
+// cell accesses (a D0 value read from a resident raster).
+static bool solo(const Group* g) {
+  if (pattern_is(g->pat, RADIAL) || pattern_is(g->pat, SPREAD)) return true;
+  for (Node* n : g->nodes)
+    if (n->kind == Kind::Access) return true;
+  return false;
+}
 
 bool Fuser::canPipe(Group* top, Group* bot) {  // Fusioner.cpp:87-96
   if (top == bot) return false;
-  if (radial(top) || radial(bot)) return false;
+  if (solo(top) || solo(bot)) return false;
   if (!same_geometry(top, bot)) return false;
   for (Group* n : top->next)
     if (n != bot && isNext(bot, n)) return false;  // a path top -> n -> ... -> bot: fusing makes a cycle
@@ -209,7 +215,7 @@ bool Fuser::canPipe(Group* top, Group* bot) {  // Fusioner.cpp:87-96
 
 bool Fuser::canFlat(Group* left, Group* right) {  // Fusioner.cpp:98-106
   if (left == right) return false;
-  if (radial(left) || radial(right)) return false;
+  if (solo(left) || solo(right)) return false;
   if (!same_geometry(left, right)) return false;
   if (isNext(right, left) || isNext(left, right)) return false;
   return can_flat_fuse(left->pat, right->pat, opt_.focal_zonal);

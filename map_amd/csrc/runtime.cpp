@@ -17,6 +17,10 @@ namespace radial {  // radial.hip
 int run(void* out, const void* in, int dt_bytes, int op, int BS0, int BS1, int NB0, int NB1, int DS0, int DS1, int s0,
         int s1, void* stream);
 }  // namespace radial
+namespace spread {  // spread.hip
+int run(void* out, const void* in, const void* dir, void* cnt, int dt, int BS0, int BS1, int NB0, int NB1, int DS0,
+        int DS1, void* stream);
+}  // namespace spread
 }  // namespace mapx
 
 namespace mapx {
@@ -62,6 +66,17 @@ void Store::release(void* p) {
   cached_ += it->second;
   in_use_ -= it->second;
   size_.erase(it);
+  if (cached_ > kCacheCap) {
+    // many distinct sizes (rasters of many geometries) would otherwise pile up in the
+    // cache: give the largest cached buffers back until it is under the cap
+    dev::sync();
+    while (cached_ > kCacheCap / 2 && !free_.empty()) {
+      auto last = std::prev(free_.end());
+      dev::free_device(last->second);
+      cached_ -= last->first;
+      free_.erase(last);
+    }
+  }
 }
 
 void Store::trim() {
@@ -574,6 +589,24 @@ static void check_focal_geometry(const Meta& m, int H0, int H1) {
     fail("edge blocks narrower than the focal halo: choose a block size whose remainder is 0 or >= the halo");
 }
 
+// Access: one cell of a resident raster (block-tiled), copied back once the
+// stream has produced it.
+Variant Runtime::access_value(Node* n) {
+  Node* p = n->prev[0];
+  const Meta& m = p->meta;
+  const int x = n->coord.arr[0], y = n->coord.arr[1];
+  const int bx = x / m.bs.arr[0], by = y / m.bs.arr[1];
+  const size_t e = ((size_t)(by * m.nb(0) + bx) * m.bs.arr[1] + (size_t)(y - by * m.bs.arr[1])) * m.bs.arr[0] +
+                   (size_t)(x - bx * m.bs.arr[0]);
+  const size_t sz = sizeof_dt(m.dt);
+  Variant v = variant_zero(m.dt);
+  PauseRecording pause;
+  dev::d2h(&v.u, (const char*)residentPtr(p) + e * sz, sz);
+  dev::sync();
+  cnt.bytes_d2h += (long)sz;
+  return v;
+}
+
 void Runtime::scalar_task(const Group& g) {
   std::vector<Node*> v(g.nodes);
   v.insert(v.end(), g.outs.begin(), g.outs.end());
@@ -584,7 +617,7 @@ void Runtime::scalar_task(const Group& g) {
   for (Node* n : v) {
     if (n->meta.nd != D0) continue;
     if (n->kind == Kind::Constant) continue;
-    n->value = eval_scalar_node(n);
+    n->value = n->kind == Kind::Access ? access_value(n) : eval_scalar_node(n);
     n->evaluated = true;
   }
   cnt.scalar_tasks++;
@@ -669,6 +702,55 @@ void Runtime::radial_task(const Group& g, Eval& e) {
   PauseRecording pause;
   cnt.launches += radial::run(out, residentPtr(n->prev[0]), sizeof_dt(m.dt), op, m.bs.arr[0], m.bs.arr[1], m.nb(0),
                               m.nb(1), m.ds.arr[0], m.ds.arr[1], n->coord.arr[0], n->coord.arr[1], dev::stream());
+}
+
+// The targets and the nodes evaluated with them: the MAX / MIN reductions of
+// every Stats node they depend on (the reference computes them in the Stats
+// task, runtime/task/StatsTask.cpp:95-160).
+std::vector<Node*> Runtime::with_aux(const std::vector<Node*>& targets) {
+  std::vector<Node*> out = targets;
+  std::set<Node*> have(targets.begin(), targets.end());
+  for (Node* n : list_upwards(targets))
+    for (Node* a : n->aux)
+      if (have.insert(a).second) out.push_back(a);
+  return out;
+}
+
+// Groups evaluated on the host: D0 chains, and cell accesses (a D0 value read
+// back from a resident raster, their input).
+bool Runtime::host_group(const Group& g) {
+  if (g.numdim() == D0) return true;
+  for (Node* n : g.nodes)
+    if (n->kind == Kind::Access) return true;
+  return false;
+}
+
+Node* Runtime::spread_node(const Group& g) {
+  for (Node* n : g.nodes)
+    if (n->kind == Kind::SpreadScan) return n;
+  return nullptr;
+}
+
+// A spreading group is its SpreadScan node alone: the accumulation of a
+// resident raster along the resident direction raster (spread.hip: one pass
+// counting upstream cells, one pass of walks from the sources), with a count
+// word per cell taken from the store for the two launches.
+void Runtime::spread_task(const Group& g, Eval& e) {
+  Node* n = spread_node(g);
+  if (g.nodes.size() != 1) fail("spread scan fused with other nodes");
+  for (Node* in : g.ins)
+    if (in->meta.nd == D2) ensure_input(in);
+  bool zonal = false;
+  if (kernel_outputs(g, e.targets, zonal).empty()) return;
+  const Meta& m = n->meta;
+  void* out = store.alloc(rasterBytes(m));
+  resident_[n] = out;
+  const size_t cells = (size_t)m.bs.arr[0] * m.bs.arr[1] * m.nb(0) * m.nb(1);
+  void* counts = store.alloc(cells * 4);
+  PauseRecording pause;
+  cnt.launches += spread::run(out, residentPtr(n->prev[0]), residentPtr(n->prev[1]), counts, m.dt, m.bs.arr[0],
+                              m.bs.arr[1], m.nb(0), m.nb(1), m.ds.arr[0], m.ds.arr[1], dev::stream());
+  store.release(counts);  // reused in stream order, after the walks
 }
 
 bool Runtime::prepare_task(const Group& g, Eval& e, Task& t) {
@@ -956,8 +1038,9 @@ void Runtime::run_levels(const std::vector<std::pair<Eval*, size_t>>& work, std:
         const Group& g = *gp;
         cnt.groups++;
         Task& t = tasks[i++];
-        if (g.numdim() == D0) { scalar_task(g); continue; }
+        if (host_group(g)) { scalar_task(g); continue; }
         if (radial_node(g)) { radial_task(g, *w.first); continue; }
+        if (spread_node(g)) { spread_task(g, *w.first); continue; }
         if (prepare_task(g, *w.first, t)) order.push_back(&t);
       }
   }
@@ -1084,7 +1167,7 @@ void Runtime::evaluate(const std::vector<Node*>& targets) {
   t.start();
 
   Eval e;
-  e.targets = targets;
+  e.targets = with_aux(targets);
   struct Uncut {
     Runtime* r; Eval& e;
     ~Uncut() { r->eval_uncut(e); }
@@ -1155,7 +1238,11 @@ std::string Runtime::plan(const std::vector<Node*>& targets) {
         << "]}}";
       continue;
     }
-    if (g->numdim() != D0) {
+    if (Node* sn = spread_node(*g)) {  // precompiled walk kernels (spread.hip)
+      j << ", \"spread\": {\"reduction\": " << sn->op << "}}";
+      continue;
+    }
+    if (!host_group(*g)) {
       std::vector<Node*> outs = d2_outputs(*g, targets), kouts;
       for (Node* n : outs)
         if (std::find(g->nodes.begin(), g->nodes.end(), n) != g->nodes.end()) kouts.push_back(n);

@@ -56,6 +56,7 @@ class Store {
   size_t cached() const { return cached_; }
   ~Store();
  private:
+  static constexpr size_t kCacheCap = size_t(32) << 30;  // bytes of freed buffers kept for reuse
   std::multimap<size_t, void*> free_;
   std::unordered_map<void*, size_t> size_;
   size_t in_use_ = 0, cached_ = 0;
@@ -80,6 +81,8 @@ class Runtime {
   void* residentPtr(Node* n);
   size_t rasterBytes(const Meta& m) const;
   static Node* radial_node(const Group& g);  // the RadialScan node of a radial group, or null
+  static Node* spread_node(const Group& g);  // the SpreadScan node of a spreading group, or null
+  static bool host_group(const Group& g);    // evaluated by scalar_task (D0 values, cell accesses)
   void fetch(Node* n, void* host, size_t bytes);  // copies a resident raster (block-tiled) to host
   void drop(Node* n);                              // releases a node's resident raster
   void adopt(Node* n, void* dev) { resident_[n] = dev; }  // raster produced in HBM by a generator
@@ -150,6 +153,9 @@ class Runtime {
   void run_levels(const std::vector<std::pair<Eval*, size_t>>& work, std::vector<Readback>* async);
   bool prepare_task(const Group& g, Eval& e, Task& t);
   void radial_task(const Group& g, Eval& e);
+  void spread_task(const Group& g, Eval& e);
+  std::vector<Node*> with_aux(const std::vector<Node*>& targets);
+  Variant access_value(Node* n);
   void run_batch(const std::vector<Task*>& batch, std::vector<Readback>* async);
   void fold_results(const std::vector<Task*>& batch, void* parts, void* tasks_dev, void* starts_dev, int maxn,
                     std::vector<Readback>* async);

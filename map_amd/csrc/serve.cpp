@@ -80,7 +80,7 @@ long long Runtime::serveSubmit(const std::vector<Node*>& targets) {
   if (targets.empty()) throw std::runtime_error("serveSubmit: nothing to evaluate");
   auto r = std::make_unique<Request>();
   r->id = srv_->next_id++;
-  r->ev.targets = targets;
+  r->ev.targets = with_aux(targets);
   for (Node* t : targets) t->ref++;  // held until the request retires
   r->t_submit = now_s();
   srv_->queued.push_back(std::move(r));
@@ -158,10 +158,14 @@ int Runtime::serveStep(int wait_us) {
     std::vector<Node*> pins;
     try {
       for (auto& g : e.plan->groups) {
-        if (g->numdim() == D0) continue;
-        if (Node* rn = radial_node(*g)) {  // no generated kernel: its output raster
+        if (host_group(*g)) continue;
+        Node* rn = radial_node(*g);
+        Node* sn = spread_node(*g);
+        if (rn || sn) {  // no generated kernel: its output raster (and a spread's count words)
           bool z = false;
-          if (!kernel_outputs(*g, e.targets, z).empty()) fp += rasterBytes(rn->meta);
+          const Meta& m = (rn ? rn : sn)->meta;
+          if (!kernel_outputs(*g, e.targets, z).empty())
+            fp += rasterBytes(m) + (sn ? (size_t)m.bs.arr[0] * m.bs.arr[1] * m.nb(0) * m.nb(1) * 4 : 0);
           for (Node* n : g->ins)
             if (n->meta.nd == D2 && resident_.count(n)) pins.push_back(n);
             else if (n->meta.nd == D2 && n->kind == Kind::Read) fp += rasterBytes(n->meta);

@@ -218,22 +218,38 @@ class Raster:
         return id(self)
 
     def __getitem__(self, key):
-        raise MapError("coordinate access is not supported by this backend")
+        """raster[x, y]: the D0 value of one cell (python/map.py:205-211); raster[mask]:
+        the masked raster, con(mask, raster, 0)."""
+        if isinstance(key, (list, tuple)):
+            return Raster(_lib.ma_access(self, Array(list(key))))
+        if isinstance(key, Raster):
+            return Raster(_lib.ma_conditional(key, self, zeros_like(self)))
+        raise MapError("a raster is indexed by a cell [x, y] or by a boolean raster")
 
     def __setitem__(self, key, other):
-        if isinstance(key, Raster):  # boolean lhs indexing: self = con(key, other, self)
-            other = _const_to_raster(other).astype(self.datatype())
-            old = self._node
-            self._node = _node(_lib.ma_conditional(key, other, self))
-            _lib.ma_increaseRef(self)
-            _lib.ma_decreaseRef(Node(old))
-            return
-        raise MapError("coordinate assignment is not supported by this backend")
+        """raster[x, y] = v replaces one cell; raster[mask] = v replaces the masked cells
+        (python/map.py:213-225)."""
+        other = _const_to_raster(other).astype(self.datatype())
+        if isinstance(key, (list, tuple)):
+            new = _node(_lib.ma_lhsAccess(self, other, Array(list(key))))
+        elif isinstance(key, Raster):
+            new = _node(_lib.ma_conditional(key, other, self))
+        else:
+            raise MapError("a raster is indexed by a cell [x, y] or by a boolean raster")
+        old = Raster()
+        old._node, self._node = self._node, new
+        _lib.ma_increaseRef(self)  # `old` drops the previous node's reference when it goes
 
-    def __call__(self, ngb):
+    def __call__(self, ngb, red=None):
+        """raster([dx, dy]): the neighbour at a fixed offset (python/map.py:227-233);
+        raster(dir, red): the reduction of the neighbours whose D8 direction in the raster
+        `dir` points at each cell (front/Raster.cpp:137-139, ma_spreadNeighbor)."""
+        if isinstance(ngb, Raster):
+            return Raster(_lib.ma_spreadNeighbor(self, ngb, SUM if red is None else red))
         if isinstance(ngb, (list, tuple)) and len(ngb) == 2 and not isinstance(ngb[0], Raster):
             return Raster(_lib.ma_neighbor(self, Array(list(ngb))))
-        raise MapError("bounded neighbourhoods are not supported by this backend")
+        raise MapError("bounded neighbourhoods are not supported: the reference has no kernel for them "
+                       "(runtime/skeleton has no BoundedNbh statement)")
 
     def nodeid(self):
         return _lib.ma_nodeid(self)
@@ -499,6 +515,35 @@ def rmin(raster, coord):
     return Raster(_lib.ma_radialScan(raster, MIN, Array(list(coord))))
 
 
+def spread_neighbor(raster, dir, red=SUM):
+    """One spreading step: per cell, red over the 3x3 neighbours whose D8 code in `dir`
+    points at it (ma_spreadNeighbor; the gather of front/Raster.cpp:583-606)."""
+    return Raster(_lib.ma_spreadNeighbor(raster, dir, red))
+
+
+def spread_scan(raster, dir, red=SUM):
+    """Accumulation along the U8 D8 directions `dir` (ma_spreadScan, SUM only): the total
+    leaving each cell towards its downstream cell (front/Raster.cpp:570-577 flowAccumulation)."""
+    return Raster(_lib.ma_spreadScan(raster, dir, red))
+
+
+def flow_accumulation(raster, dir):
+    return spread_scan(raster, dir, SUM)
+
+
+def barrier(raster):
+    """The raster's values; fusion never crosses it (ma_barrier)."""
+    return Raster(_lib.ma_barrier(raster))
+
+
+def stats_range(s):
+    """(max, min) of an evaluated stats(raster) node."""
+    mx, mn = Variant(), Variant()
+    if _lib.ma_statsRange(s, ct.byref(mx), ct.byref(mn)) != 0:
+        _err(None)
+    return mx.value(), mn.value()
+
+
 def zsum(raster):
     return Raster(_lib.ma_zonalReduc(raster, SUM))
 
@@ -717,8 +762,13 @@ def tune_file(path, save=False):
     return r
 
 
-def stats():
-    return _json.loads(_lib.ma_stats().decode())
+def stats(raster=None):
+    """stats(raster): the raster with its maximum and minimum computed alongside
+    (ma_stats; read them with stats_range). stats(): the runtime's counters and
+    device-memory figures as a dict (ma_runStats)."""
+    if raster is not None:
+        return Raster(_lib.ma_stats(raster))
+    return _json.loads(_lib.ma_runStats().decode())
 
 
 def last_eval_ms():
@@ -836,7 +886,14 @@ _proto('ma_setOption', [ct.c_char_p, ct.c_longlong], ct.c_int)
 _proto('ma_getOption', [ct.c_char_p], ct.c_longlong)
 _proto('ma_plan', [ct.POINTER(Node), ct.c_int], ct.c_char_p)
 _proto('ma_compilePlan', [ct.POINTER(Node), ct.c_int], ct.c_int)
-_proto('ma_stats', [], ct.c_char_p)
+_proto('ma_runStats', [], ct.c_char_p)
+_proto('ma_stats', [Raster], Node)
+_proto('ma_statsRange', [Raster, ct.POINTER(Variant), ct.POINTER(Variant)], ct.c_int)
+_proto('ma_barrier', [Raster], Node)
+_proto('ma_access', [Raster, Array], Node)
+_proto('ma_lhsAccess', [Raster, Raster, Array], Node)
+_proto('ma_spreadNeighbor', [Raster, Raster, ct.c_int], Node)
+_proto('ma_spreadScan', [Raster, Raster, ct.c_int], Node)
 _proto('ma_lastEvalMs', [], ct.c_double)
 _proto('ma_kernelTimes', [], ct.c_char_p)
 _proto('ma_kernelSource', [ct.c_char_p], ct.c_char_p)

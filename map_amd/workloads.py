@@ -115,3 +115,43 @@ def fit_side(side, block, halo):
     thinner last block grow by the few missing cells."""
     r = side % block
     return side if r == 0 or r >= halo else side + (halo - r)
+
+
+# D8 codes of the eight neighbours at offset (dx, dy), y growing south (ESRI order:
+# 1 E, 2 SE, 4 S, 8 SW, 16 W, 32 NW, 64 N, 128 NE), the codes the reference's spreading
+# kernel and flowAccuGather read (runtime/skeleton/util.cpp:628, front/Raster.cpp:586-589).
+D8_CODES = [((1, 0), 1), ((1, 1), 2), ((0, 1), 4), ((-1, 1), 8), ((-1, 0), 16), ((-1, -1), 32),
+            ((0, -1), 64), ((1, -1), 128)]
+
+
+def d8_directions(dem):
+    """Steepest-descent D8 flow directions of a DEM (rows = y) as a U8 raster: the code of the
+    neighbour with the largest drop per unit distance (the first in D8_CODES order on ties),
+    0 where no neighbour inside the raster is lower (pits and flats)."""
+    import numpy as np
+    z = np.asarray(dem, np.float64)
+    h, w = z.shape
+    best = np.zeros((h, w))
+    code = np.zeros((h, w), np.uint8)
+    pad = np.pad(z, 1, constant_values=np.inf)
+    for (dx, dy), c in D8_CODES:
+        nb = pad[1 + dy:1 + dy + h, 1 + dx:1 + dx + w]
+        drop = (z - nb) / (np.sqrt(2.0) if dx and dy else 1.0)
+        take = drop > best
+        best = np.where(take, drop, best)
+        code = np.where(take, np.uint8(c), code)
+    return code
+
+
+def flow_gather(M, water, dirs):
+    """One gathering step written with neighbour and conditional ops (the loop body of the
+    reference's flowAccuGather, front/Raster.cpp:595-602): the sum of the neighbours' water
+    whose direction points at each cell."""
+    inflow = [[2, 4, 8], [1, 0, 16], [128, 64, 32]]
+    new = M.zeros_like(water)
+    for y in (-1, 0, 1):
+        for x in (-1, 0, 1):
+            if x == 0 and y == 0:
+                continue
+            new = M.con(dirs([x, y]) == inflow[y + 1][x + 1], new + water([x, y]), new)
+    return new

@@ -533,6 +533,181 @@ def _zred(x, red):
     return R(np.array(r, dtype=np.uint64).astype(NP[dt]), dt)
 
 
+# ---- spreading (D8 flow directions) -----------------------------------------------------
+# The code of the neighbour at offset (i0, i1) of a cell when that neighbour flows into the
+# cell, indexed [i1 + 1][i0 + 1] (dircod of runtime/skeleton/util.cpp:628 and
+# front/Raster.cpp:586-589); the downstream offset of each code follows from it.
+D8_INFLOW = [[2, 4, 8], [1, 0, 16], [128, 64, 32]]
+D8_OUT = {D8_INFLOW[i1 + 1][i0 + 1]: (-i0, -i1) for i1 in (-1, 0, 1) for i0 in (-1, 0, 1) if i0 or i1}
+
+
+def _neutral(red, dt):
+    t = _work(NP[dt])
+    if red == SUM:
+        return t(0)
+    if red == PROD:
+        return t(1)
+    if dt in FLOATS:
+        return t(-np.inf) if red == MAX else t(np.inf)
+    info = np.iinfo(NP[dt]) if dt != B8 else np.iinfo(np.uint8)
+    if red in (MAX, rOR):
+        return t(info.min if dt != B8 else 0)
+    return t(info.max if dt != B8 else 1)  # MIN, rAND
+
+
+def _red_step(acc, v, red, dt):
+    """acc = OP(acc, v) in the node's C type (the statements of runtime/skeleton/Skeleton.cpp)."""
+    if dt in FLOATS:
+        if red == MAX:
+            return np.fmax(acc, v)
+        if red == MIN:
+            return np.fmin(acc, v)
+        return (acc + v) if red == SUM else (acc * v)
+    if red == MAX:
+        return np.maximum(acc, v)
+    if red == MIN:
+        return np.minimum(acc, v)
+    wide = np.uint64 if dt == U64 else np.int64
+    a, b = np.asarray(acc).astype(wide), np.asarray(v).astype(wide)
+    with np.errstate(over='ignore'):
+        r = a + b if red == SUM else a * b if red == PROD else a & b if red == rAND else a | b
+    return _conv(r, dt)
+
+
+def spread_neighbor(x, d, red=SUM):
+    """SpreadNeighbor (runtime/dag/SpreadNeighbor.cpp:31-47): per cell, red over the 3x3
+    neighbours (y outer, x inner, centre skipped) whose code in d points at the cell, from the
+    reduction's neutral value, with the focal border rule; the gather of flowAccuGather
+    (front/Raster.cpp:583-606) as one node."""
+    ds = x.datasize()
+    idx, nt = taps(ds, x.bs, 1, 1)
+    g = idx.reshape(-1, nt)
+    src = np.asarray(x.val).reshape(-1)
+    dsrc = np.asarray(d.val).reshape(-1)
+    acc = np.full(src.shape, _neutral(red, x.dt), dtype=src.dtype)
+    for t in range(9):
+        i1, i0 = t // 3 - 1, t % 3 - 1
+        if i0 == 0 and i1 == 0:
+            continue
+        sel = dsrc[g[:, t]] == D8_INFLOW[i1 + 1][i0 + 1]
+        acc = np.where(sel, _red_step(acc, src[g[:, t]], red, x.dt), acc).astype(src.dtype)
+    return R(acc.reshape(ds[1], ds[0]), x.dt, x.bs)
+
+
+def _d8_downstream(d):
+    """Downstream cell of every cell of a direction raster (rows = y) and whether it has one."""
+    h, w = d.shape
+    yy, xx = np.mgrid[0:h, 0:w]
+    tx, ty = xx.copy(), yy.copy()
+    has = np.zeros((h, w), bool)
+    for code, (ox, oy) in D8_OUT.items():
+        m = d == code
+        tx[m] += ox
+        ty[m] += oy
+        has |= m
+    has &= (tx >= 0) & (tx < w) & (ty >= 0) & (ty < h)
+    return tx, ty, has
+
+
+def spread_scan(x, d, red=SUM):
+    """SpreadScan with SUM (runtime/dag/SpreadScan.cpp; SpreadingSkeleton.cpp:288-388): with
+    T(c) = in(c) + the sum of T(u) over the upstream neighbours u of c (the neighbours whose
+    code points at c, y outer, x inner; in the raster's C type), a cell with a downstream cell
+    inside the raster gets T(c) (what leaves it), any other cell 0; cells on a direction cycle
+    or downstream of one never have all their upstream totals and get 0 (the reference's
+    iteration does not end for them). Evaluated in topological order (Kahn's algorithm), one
+    front of ready cells at a time."""
+    if red != SUM:
+        raise NotImplementedError("spread_scan: SUM only")
+    dv = np.asarray(d.val)
+    h, w = dv.shape
+    v = np.asarray(x.val)
+    tx, ty, has = _d8_downstream(dv)
+    left = np.zeros((h, w), np.int64)
+    np.add.at(left, (ty[has], tx[has]), 1)
+    tot = np.zeros_like(v)
+    done = np.zeros((h, w), bool)
+    ry, rx = np.nonzero(left == 0)
+    while ry.size:
+        acc = v[ry, rx].copy()
+        for i1 in (-1, 0, 1):
+            for i0 in (-1, 0, 1):
+                if i0 == 0 and i1 == 0:
+                    continue
+                ny, nx = ry + i1, rx + i0
+                ok = (nx >= 0) & (nx < w) & (ny >= 0) & (ny < h)
+                nyc, nxc = np.clip(ny, 0, h - 1), np.clip(nx, 0, w - 1)
+                ok &= dv[nyc, nxc] == D8_INFLOW[i1 + 1][i0 + 1]
+                acc = np.where(ok, _red_step(acc, tot[nyc, nxc], SUM, x.dt), acc).astype(v.dtype)
+        tot[ry, rx] = acc
+        done[ry, rx] = True
+        go = has[ry, rx]
+        gy, gx = ty[ry, rx][go], tx[ry, rx][go]
+        np.subtract.at(left, (gy, gx), 1)
+        nxt = np.unique(gy * w + gx)
+        nxt = nxt[left.reshape(-1)[nxt] == 0]
+        ry, rx = nxt // w, nxt % w
+    out = np.where(has & done, tot, np.zeros_like(tot)).astype(v.dtype)
+    return R(out, x.dt, x.bs)
+
+
+def flow_accumulation(x, d):
+    return spread_scan(x, d, SUM)
+
+
+def stats(x):
+    """Stats (runtime/dag/Stats.cpp): the raster's values, with its maximum and minimum."""
+    r = R(np.asarray(x.val).copy(), x.dt, x.bs)
+    r.range = (zmax(x), zmin(x))
+    return r
+
+
+def stats_range(s):
+    return value(s.range[0]), value(s.range[1])
+
+
+def barrier(x):
+    return R(np.asarray(x.val).copy(), x.dt, x.bs)
+
+
+def access(x, cell):
+    """Access (runtime/dag/Access.cpp): the value of cell [x, y]."""
+    return R(np.asarray(x.val)[cell[1], cell[0]].copy(), x.dt)
+
+
+def lhs_access(x, v, cell):
+    """LhsAccess (Skeleton.cpp:290-297): x with cell [x, y] set to v in x's type."""
+    out = np.asarray(x.val).copy()
+    out[cell[1], cell[0]] = np.asarray(_conv(np.asarray(const(v).val), x.dt))
+    return R(out, x.dt, x.bs)
+
+
+def zeros_like(x, dt=None):
+    dt = x.dt if dt is None else dt
+    return R(np.zeros(np.asarray(x.val).shape, _work(NP[dt])), dt, x.bs)
+
+
+def _r_getitem(self, key):
+    if isinstance(key, (list, tuple)):
+        return access(self, key)
+    return con(key, self, const(0).astype(self.dt))
+
+
+def _r_setitem(self, key, other):
+    other = const(other).astype(self.dt)
+    new = lhs_access(self, other, key) if isinstance(key, (list, tuple)) else con(key, other, self)
+    self.val, self.dt = new.val, new.dt
+
+
+def _r_call(self, ngb, red=None):
+    if isinstance(ngb, R):
+        return spread_neighbor(self, ngb, SUM if red is None else red)
+    return neighbor(self, ngb)
+
+
+R.__getitem__, R.__setitem__, R.__call__ = _r_getitem, _r_setitem, _r_call
+
+
 def zsum(x):
     return _zred(x, SUM)
 

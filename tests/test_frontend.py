@@ -165,3 +165,37 @@ def test_radial_scan_front_end_checks():
     groups = M.plan(v)["groups"]
     radial = [g for g in groups if "radial" in g]
     assert len(radial) == 1 and radial[0]["kinds"] == ["RadialScan"] and radial[0]["radial"]["start"] == [10, 20]
+
+
+def test_spreading_front_end_checks_and_plans(M):
+    """Spreading scans need U8 directions and SUM, and run as a group of their own; a
+    spreading step, Stats, Barrier and LhsAccess compile as generated kernels for gfx950."""
+    f = M.from_array(np.zeros((40, 60), np.float32), (32, 32))
+    d = M.from_array(np.zeros((40, 60), np.uint8), (32, 32))
+    with pytest.raises(M.MapError, match="U8"):
+        M.spread_scan(f, M.from_array(np.zeros((40, 60), np.int32), (32, 32)))
+    with pytest.raises(M.MapError, match="SUM"):
+        M.spread_scan(f, d, M.MAX)
+    with pytest.raises(M.MapError, match="shape"):
+        M.spread_scan(f, M.from_array(np.zeros((40, 61), np.uint8), (32, 32)))
+    v = M.spread_scan(f * 2.0, d) + 1.0
+    spread = [g for g in M.plan(v)["groups"] if "spread" in g]
+    assert len(spread) == 1 and spread[0]["kinds"] == ["SpreadScan"]
+    c = f * 1.0
+    c[[3, 4]] = 2.5
+    for e in (f(d) * 2.0, M.spread_neighbor(f, d, M.MAX), M.stats(f + 1.0), M.barrier(f) * 2.0, c):
+        assert M.compile_plan(e) >= 1
+    # a Barrier keeps its producer and consumer in two kernels
+    assert len([g for g in M.plan(M.barrier(f + 1.0) * 2.0)["groups"] if "kernel" in g]) == 2
+
+
+def test_library_exports_every_declared_entry_point():
+    """libmapx.so exports every function include/mapx.h declares (loaded without a GPU)."""
+    import ctypes
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    names = re.findall(r"\b(ma_\w+)\s*\(", open(os.path.join(root, "include", "mapx.h")).read())
+    lib = ctypes.CDLL(os.path.join(root, "map_amd", "libmapx.so"))
+    missing = [n for n in sorted(set(names)) if not hasattr(lib, n)]
+    assert len(set(names)) > 50 and not missing, missing

@@ -1,0 +1,160 @@
+"""Pins the oracle's spreading functions (spread_neighbor, spread_scan) and the
+small nodes around them (stats, access, lhs_access) without a GPU.
+
+spread_scan is checked against brute force (every cell's flow path followed to its
+end: the total that leaves a cell is the sum of the inputs whose path passes
+through it), against hand-worked cases, and its FP32 order cell by cell;
+spread_neighbor against the reference's own formulation of the gather step with
+neighbour and conditional ops (front/Raster.cpp:583-606).
+"""
+import numpy as np
+import pytest
+
+from tests import scripts
+
+
+def _brute_scan(v, d, mod=None):
+    """Sum of v over every cell whose D8 path passes through each cell (Python ints),
+    0 for cells without a downstream cell and for cells reached from a cycle."""
+    h, w = d.shape
+    off = {c: o for o, c in scripts.D8_CODES}
+    nxt = {}
+    for y in range(h):
+        for x in range(w):
+            o = off.get(int(d[y, x]))
+            if o is not None and 0 <= x + o[0] < w and 0 <= y + o[1] < h:
+                nxt[(x, y)] = (x + o[0], y + o[1])
+    # cells on a cycle, and every cell downstream of one, never finish
+    on_cycle = set()
+    for start in nxt:
+        seen, c = [], start
+        while c in nxt and c not in seen:
+            seen.append(c)
+            c = nxt[c]
+        if c in seen:
+            on_cycle.update(seen[seen.index(c):])
+    bad = set(on_cycle)
+    for start in on_cycle:
+        c = start
+        while c in nxt:
+            c = nxt[c]
+            if c in bad and c != start:
+                break
+            bad.add(c)
+    tot = {}
+    for y in range(h):
+        for x in range(w):
+            c = (x, y)
+            path = [c]
+            while c in nxt and len(path) <= h * w:
+                c = nxt[c]
+                path.append(c)
+            for p in set(path):
+                tot[p] = tot.get(p, 0) + int(v[y, x])
+    out = np.zeros((h, w), np.int64)
+    for (x, y), t in tot.items():
+        if (x, y) in nxt and (x, y) not in bad:
+            out[y, x] = t if mod is None else t % mod
+    return out
+
+
+def test_d8_codes_point_the_stated_way():
+    """1 east, 4 south (y grows down), 16 west, 64 north; the oracle's inflow table is the
+    opposite offset of each code (a transposed table fails here)."""
+    from oracle import ref as O
+    assert O.D8_OUT[1] == (1, 0) and O.D8_OUT[4] == (0, 1) and O.D8_OUT[16] == (-1, 0) and O.D8_OUT[64] == (0, -1)
+    assert O.D8_OUT[2] == (1, 1) and O.D8_OUT[8] == (-1, 1) and O.D8_OUT[32] == (-1, -1) and O.D8_OUT[128] == (1, -1)
+    assert dict((c, o) for o, c in scripts.D8_CODES) == O.D8_OUT
+
+
+def test_spread_scan_hand_worked(O):
+    """Rows 0-1 flow east into column 3, column 3 flows south, row 2 flows north; the cell
+    at (3, 2) has no code: it is a sink and keeps 0; what leaves each cell is counted."""
+    d = np.array([[1, 1, 1, 4], [1, 1, 1, 4], [64, 64, 64, 0]], np.uint8)
+    x = O.from_array(np.ones((3, 4), np.int32), (2, 2))
+    got = O.spread_scan(x, O.from_array(d, (2, 2))).val
+    assert np.array_equal(got, [[1, 2, 3, 4], [2, 4, 6, 11], [1, 1, 1, 0]])
+    # a line flowing east: the last cell flows off the raster and keeps 0
+    line = O.spread_scan(O.from_array(np.arange(1, 7, dtype=np.int32)[None, :], (4, 1)),
+                         O.from_array(np.ones((1, 6), np.uint8), (4, 1))).val
+    assert np.array_equal(line, [[1, 3, 6, 10, 15, 0]])
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_spread_scan_equals_brute_force(O, seed):
+    """DEM-derived directions (with pits) and random codes (with invalid codes and cycles)."""
+    rng = np.random.default_rng(seed)
+    h, w = 23, 31
+    dem = rng.normal(size=(h, w)).cumsum(0).cumsum(1)
+    for d in (scripts.d8_directions(dem),
+              rng.choice(np.array([0, 1, 2, 4, 8, 16, 32, 64, 128, 3, 255], np.uint8), (h, w))):
+        v = rng.integers(0, 50, (h, w)).astype(np.int32)
+        got = O.spread_scan(O.from_array(v, (8, 8)), O.from_array(d, (8, 8))).val
+        assert np.array_equal(got, _brute_scan(v, d))
+        # U8 totals wrap as C's unsigned char arithmetic does
+        v8 = rng.integers(0, 256, (h, w)).astype(np.uint8)
+        got8 = O.spread_scan(O.from_array(v8, (8, 8)), O.from_array(d, (8, 8))).val
+        assert got8.dtype == np.uint8 and np.array_equal(got8, _brute_scan(v8, d, 256))
+
+
+def test_spread_scan_fp32_order_cell_by_cell(O):
+    """Every cell with a downstream cell holds fl32(in + T(u1) + T(u2) ...) over its upstream
+    neighbours in y-outer, x-inner order, rounded after each addition."""
+    rng = np.random.default_rng(5)
+    h, w = 40, 37
+    d = scripts.d8_directions(rng.normal(size=(h, w)).cumsum(0).cumsum(1))
+    v = rng.uniform(0, 1, (h, w)).astype(np.float32)
+    out = O.spread_scan(O.from_array(v, (16, 16)), O.from_array(d, (16, 16))).val
+    _, _, has = O._d8_downstream(d)
+    for y in range(h):
+        for x in range(w):
+            if not has[y, x]:
+                assert out[y, x] == 0
+                continue
+            acc = np.float32(v[y, x])
+            for i1 in (-1, 0, 1):
+                for i0 in (-1, 0, 1):
+                    ny, nx = y + i1, x + i0
+                    if (i0 or i1) and 0 <= nx < w and 0 <= ny < h and d[ny, nx] == O.D8_INFLOW[i1 + 1][i0 + 1]:
+                        acc = np.float32(acc + out[ny, nx])
+            assert out[y, x] == acc, (x, y)
+    # and the FP32 totals stay within rounding of the exact sums
+    exact = _brute_scan((v * 2**24).astype(np.int64), d) / 2**24
+    assert np.allclose(out, exact, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.float32, np.uint8])
+def test_spread_neighbor_equals_the_reference_gather(O, dt):
+    """spread_neighbor(SUM) is the body of flowAccuGather (neighbour reads compared with the
+    inflow codes, conditional sums) bit for bit, border rule included (blocks 16x8 on a
+    ragged 45x30 raster)."""
+    rng = np.random.default_rng(9)
+    h, w, bs = 30, 45, (16, 8)
+    d = scripts.d8_directions(rng.normal(size=(h, w)).cumsum(0).cumsum(1))
+    v = rng.integers(0, 9, (h, w)).astype(dt) if dt != np.float32 else rng.uniform(0, 1, (h, w)).astype(dt)
+    X, D = O.from_array(v, bs), O.from_array(d, bs)
+    assert np.array_equal(O.spread_neighbor(X, D).val, scripts.flow_gather(O, X, D).val)
+    assert np.array_equal(X(D).val, O.spread_neighbor(X, D, O.SUM).val)
+
+
+def test_spread_neighbor_max_and_hand_case(O):
+    """MAX picks the largest inflowing value, the neutral value where none flows in; at a raster
+    edge the mirrored neighbour's code counts (the focal border rule)."""
+    d = np.array([[1, 4, 0], [0, 0, 16]], np.uint8)
+    v = np.array([[5, 7, 1], [2, 3, 9]], np.int32)
+    got = O.spread_neighbor(O.from_array(v, (3, 2)), O.from_array(d, (3, 2)), O.MAX).val
+    lo = np.iinfo(np.int32).min
+    # (1,0) gets 5 from (0,0) [E]; (1,1) gets 7 from (1,0) [S] and 9 from (2,1) [W];
+    # (0,0) sees the mirror of itself at (-1,0) with code E: 5; (1,0) also sees (1,-1) = itself (S): 7;
+    # (2,1) sees the mirror of itself at (3,1) with code W: 9
+    assert got.tolist() == [[5, 7, lo], [lo, 9, 9]]
+
+
+def test_stats_access_lhs_access(O):
+    a = np.arange(12, dtype=np.int32).reshape(3, 4) - 5
+    X = O.from_array(a, (2, 2))
+    s = O.stats(X)
+    assert np.array_equal(s.val, a) and O.stats_range(s) == (6, -5)
+    assert O.value(X[[3, 1]]) == a[1, 3]
+    X[[2, 0]] = 100
+    assert X.val[0, 2] == 100 and X.val.sum() == a.sum() - a[0, 2] + 100
