@@ -377,6 +377,39 @@ def radial_bench(M, side=16384, block=1024, reps=3):
     return {"workload": "radial scan from the centre of a %dx%d f32 DEM, blocks %dx%d" % (side, side, block, block), **out}
 
 
+def spread_bench(M, side=16384, block=1024, reps=3):
+    """Flow accumulation (ma_spreadScan, SUM) of an F32 raster of ones along the steepest-descent
+    D8 directions of a side^2 DEM (directions computed on the device by a fused focal script).
+    Device time of the scan from HIP events; algorithmic traffic 9 B/cell (F32 input and
+    U8 direction read, F32 result written)."""
+    import numpy as np
+    from map_amd import workloads as W
+    rng = np.random.Generator(np.random.PCG64(6))
+    dem = M.gen_dem((side, side), (block, block), W.dem_gaussians(rng, side, side, ref_side=4096.0 * side / 4096), 6)
+    dirs = W.d8_script(M, dem)
+    water = M.ones_like(dem)
+    M.eval(dirs, water)
+    acc = M.flow_accumulation(water, dirs)
+    M.eval(acc)
+    devs = []
+    for _ in range(reps):
+        M.release(acc)
+        M.eval(acc)
+        devs.append(M.last_eval_ms())
+    ms = min(devs)
+    top = M.zmax(acc)
+    M.eval(top)
+    gbs = 9.0 * side * side / (ms * 1e-3) / 1e9
+    res = {"workload": "flow accumulation of ones along D8 steepest descent, %dx%d f32 DEM, blocks %dx%d"
+                       % (side, side, block, block),
+           "device_ms": round(ms, 3), "gcells_s": round(side * side / (ms * 1e-3) / 1e9, 2),
+           "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+           "largest_total": M.value(top)}
+    del acc, top, water, dirs
+    M.release(dem)
+    return res
+
+
 def hillshade_bench(M, side=4096, block=512, reps=5):
     """The plain hillshade script (python/hill.py) on a side^2 FP32 DEM in
     block^2 blocks: one fused focal+local group, plain and batched forms."""
@@ -420,7 +453,7 @@ def main():
     ap.add_argument("--tile", default="", help="TX,TY,VX[,PF] tile shape of the Life kernel (implies --tune 0)")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the Life headline (skip hillshade, hillshade+zonal, batched tiles, serving)")
-    ap.add_argument("--only", choices=["hillshade", "hillshade_zonal", "batched_tiles", "serve", "radial"],
+    ap.add_argument("--only", choices=["hillshade", "hillshade_zonal", "batched_tiles", "serve", "radial", "spread"],
                     help="run one extra section alone (no Life headline), print its JSON")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a U8 copy of the input raster (a local cast kernel), for counter calibration")
@@ -433,7 +466,8 @@ def main():
     M.setupDevices("", M.DEV_GPU, "")
     if args.only:
         fn = {"hillshade": hillshade_bench, "hillshade_zonal": hillshade_zonal_bench,
-              "batched_tiles": batched_tiles_bench, "serve": serve_bench, "radial": radial_bench}[args.only]
+              "batched_tiles": batched_tiles_bench, "serve": serve_bench, "radial": radial_bench,
+              "spread": spread_bench}[args.only]
         print(json.dumps({args.only: fn(M)}), flush=True)
         return
     bs = (args.block, args.block)
@@ -562,7 +596,8 @@ def main():
         M.set_option("graph", 0)
         extras = {}
         for name, fn in (("hillshade", hillshade_bench), ("hillshade_zonal", hillshade_zonal_bench),
-                         ("batched_tiles", batched_tiles_bench), ("serve", serve_bench), ("radial", radial_bench)):
+                         ("batched_tiles", batched_tiles_bench), ("serve", serve_bench), ("radial", radial_bench),
+                         ("spread", spread_bench)):
             t0 = time.perf_counter()
             try:
                 extras[name] = fn(M)

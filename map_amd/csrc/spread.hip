@@ -102,6 +102,17 @@ __global__ void __launch_bounds__(NT) k_init(const uint8_t* __restrict__ dir, un
   out[e] = T(0);
 }
 
+// Totals move between walks through the device's coherence point: agent-scope
+// loads and stores (they bypass the XCD-local L2 caches), ordered by waiting for the
+// store to complete before the count is decremented. A release / acquire fence pair
+// would do the same by writing back and invalidating a whole L2 at every step.
+template <class T> __device__ __forceinline__ T ld_total(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ void st_total(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <class T>
 __global__ void __launch_bounds__(NT) k_walk(const T* __restrict__ in, const uint8_t* __restrict__ dir,
                                              unsigned* cnt, T* out, Geo g, uint64_t n) {
@@ -112,25 +123,41 @@ __global__ void __launch_bounds__(NT) k_walk(const T* __restrict__ in, const uin
   cell_of(g, e0, x, y);
   uint64_t e = e0;
   for (;;) {
-    T acc = in[e];
-    for (int i1 = -1; i1 <= 1; i1++)
-      for (int i0 = -1; i0 <= 1; i0++) {
-        const int u0 = x + i0, u1 = y + i1;
-        if ((i0 | i1) == 0 || u0 < 0 || u1 < 0 || u0 >= g.DS0 || u1 >= g.DS1) continue;
-        const uint64_t eu = at(g, u0, u1);
-        if (dir[eu] == inflow_code(i0, i1)) acc = (T)(acc + out[eu]);
-      }
+    // one round of loads for the cell and its eight neighbours (clamped into the raster;
+    // a neighbour outside it never matches), then one round for the upstream totals
+    uint64_t en[9];
+    bool inr[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const int u0 = x + k % 3 - 1, u1 = y + k / 3 - 1;
+      inr[k] = u0 >= 0 && u1 >= 0 && u0 < g.DS0 && u1 < g.DS1;
+      en[k] = inr[k] ? at(g, u0, u1) : e;
+    }
+    unsigned dn[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) dn[k] = dir[en[k]];
+    const T own = in[e];
+    bool up[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) up[k] = k != 4 && inr[k] && dn[k] == inflow_code(k % 3 - 1, k / 3 - 1);
+    T tv[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) tv[k] = ld_total(&out[up[k] ? en[k] : e]);
+    T acc = own;
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+      if (up[k]) acc = (T)(acc + tv[k]);
     int dx, dy;
-    const int x1 = x, y1 = y;
-    if (!outflow(dir[e], dx, dy)) return;  // no downstream code: the neutral value stays
-    x = x1 + dx;
-    y = y1 + dy;
+    if (!outflow(dn[4], dx, dy)) return;  // no downstream code: the neutral value stays
+    x += dx;
+    y += dy;
     if (x < 0 || y < 0 || x >= g.DS0 || y >= g.DS1) return;  // flows off the raster
-    out[e] = acc;
+    st_total(&out[e], acc);
     const uint64_t ed = at(g, x, y);
-    // release: this total is visible to the walk that takes the count to zero;
-    // acquire: that walk then reads every upstream total of the cell
-    const unsigned before = __hip_atomic_fetch_add(&cnt[ed], 0xffffffffu, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    // the total has reached the coherence point before the count falls, so the walk
+    // that takes the count to zero reads it (its loads are issued after the atomic returns)
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    const unsigned before = __hip_atomic_fetch_add(&cnt[ed], 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (before != 1u) return;
     e = ed;
   }

@@ -155,3 +155,18 @@ def flow_gather(M, water, dirs):
                 continue
             new = M.con(dirs([x, y]) == inflow[y + 1][x + 1], new + water([x, y]), new)
     return new
+
+
+def d8_script(M, dem):
+    """Steepest-descent D8 directions as a map-algebra script (one fused focal group): the code
+    of the neighbour with the largest drop per unit distance, the first in D8_CODES order on
+    ties, 0 where no neighbour is lower; neighbours outside the raster follow the focal border
+    rule (d8_directions treats them as higher instead)."""
+    best = M.zeros_like(dem)
+    code = M.zeros_like(dem, M.S32)
+    for (dx, dy), c in D8_CODES:
+        drop = (dem - dem([dx, dy])) / (1.4142135 if dx and dy else 1.0)
+        take = drop > best
+        best = M.con(take, drop, best)
+        code = M.con(take, c, code)
+    return M.astype(code, M.U8)
