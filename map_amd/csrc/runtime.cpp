@@ -732,9 +732,9 @@ Node* Runtime::spread_node(const Group& g) {
 }
 
 // A spreading group is its SpreadScan node alone: the accumulation of a
-// resident raster along the resident direction raster (spread.hip: one pass
-// counting upstream cells, one pass of walks from the sources), with a count
-// word per cell taken from the store for the two launches.
+// resident raster along the resident direction raster (spread.hip: tiles in
+// LDS, a counting pass, walks from the listed sources), with a status word per
+// cell and a source list taken from the store for the three launches.
 void Runtime::spread_task(const Group& g, Eval& e) {
   Node* n = spread_node(g);
   if (g.nodes.size() != 1) fail("spread scan fused with other nodes");
@@ -746,7 +746,7 @@ void Runtime::spread_task(const Group& g, Eval& e) {
   void* out = store.alloc(rasterBytes(m));
   resident_[n] = out;
   const size_t cells = (size_t)m.bs.arr[0] * m.bs.arr[1] * m.nb(0) * m.nb(1);
-  void* counts = store.alloc(cells * 4);
+  void* counts = store.alloc(cells * 8 + 8);  // a status word per cell, then the list of walk sources
   PauseRecording pause;
   cnt.launches += spread::run(out, residentPtr(n->prev[0]), residentPtr(n->prev[1]), counts, m.dt, m.bs.arr[0],
                               m.bs.arr[1], m.nb(0), m.nb(1), m.ds.arr[0], m.ds.arr[1], dev::stream());

@@ -165,7 +165,7 @@ int Runtime::serveStep(int wait_us) {
           bool z = false;
           const Meta& m = (rn ? rn : sn)->meta;
           if (!kernel_outputs(*g, e.targets, z).empty())
-            fp += rasterBytes(m) + (sn ? (size_t)m.bs.arr[0] * m.bs.arr[1] * m.nb(0) * m.nb(1) * 4 : 0);
+            fp += rasterBytes(m) + (sn ? (size_t)m.bs.arr[0] * m.bs.arr[1] * m.nb(0) * m.nb(1) * 8 + 8 : 0);
           for (Node* n : g->ins)
             if (n->meta.nd == D2 && resident_.count(n)) pins.push_back(n);
             else if (n->meta.nd == D2 && n->kind == Kind::Read) fp += rasterBytes(n->meta);
