@@ -16,9 +16,10 @@
 // in three passes:
 //   k_tile: a work-group per 64x64 tile computes, in LDS, every cell whose
 //     upstream cells all lie in the tile ("closed" cells: their totals need
-//     nothing from outside), in rounds of cells whose upstream is done, and
-//     marks the others "open" (a cell with an upstream neighbour outside the
-//     tile, or downstream of an open cell, or on a direction cycle);
+//     nothing from outside), by walks from the tile's sources with the
+//     hand-off described below, and marks the others "open" (a cell with an
+//     upstream neighbour outside the tile, or downstream of an open cell, or
+//     on a direction cycle);
 //   k_count: every open cell counts its open upstream neighbours and lists the
 //     sources (open cells with none);
 //   k_walk: a persistent grid walks from the listed sources. A walk computes T(c) from the finished
@@ -49,10 +50,38 @@ struct Geo {
   int BS0, BS1, NB0, NB1, DS0, DS1;
 };
 
-// block-tiled element index of cell (x, y)
-__device__ __forceinline__ uint64_t at(const Geo& g, int x, int y) {
-  const int bx = x / g.BS0, by = y / g.BS1;
-  return ((uint64_t)(by * g.NB0 + bx) * g.BS1 + (uint64_t)(y - by * g.BS1)) * g.BS0 + (uint64_t)(x - bx * g.BS0);
+// The block-tiled element index of cell (x, y) splits into a row term and a column
+// term: ((by*NB0 + bx)*BS1 + ly)*BS0 + lx = rowt(y) + colt(x), with by, ly = y / BS1,
+// y % BS1 and bx, lx = x / BS0, x % BS0.
+__device__ __forceinline__ uint64_t rowt(const Geo& g, int y) {
+  const int by = y / g.BS1;
+  return ((uint64_t)by * g.NB0 * g.BS1 + (uint64_t)(y - by * g.BS1)) * g.BS0;
+}
+__device__ __forceinline__ uint64_t colt(const Geo& g, int x) {
+  const int bx = x / g.BS0;
+  return (uint64_t)bx * g.BS1 * g.BS0 + (uint64_t)(x - bx * g.BS0);
+}
+__device__ __forceinline__ uint64_t at(const Geo& g, int x, int y) { return rowt(g, y) + colt(g, x); }
+
+// Element indices of the 3x3 neighbourhood of an inside cell (x, y), k = (i1+1)*3 + (i0+1);
+// a neighbour outside the raster gets the cell's own index and inr = false. One
+// division pair per axis; the neighbouring rows and columns follow by carries.
+__device__ __forceinline__ void hood(const Geo& g, int x, int y, uint64_t (&en)[9], bool (&inr)[9]) {
+  const int by = y / g.BS1, ly = y - by * g.BS1, bx = x / g.BS0, lx = x - bx * g.BS0;
+  const uint64_t blk = (uint64_t)g.BS1 * g.BS0, row = (uint64_t)g.NB0 * blk;
+  uint64_t r[3], c[3];
+  r[1] = (uint64_t)by * row + (uint64_t)ly * g.BS0;
+  r[0] = ly > 0 ? r[1] - g.BS0 : r[1] - row + (uint64_t)(g.BS1 - 1) * g.BS0;
+  r[2] = ly + 1 < g.BS1 ? r[1] + g.BS0 : r[1] + row - (uint64_t)(g.BS1 - 1) * g.BS0;
+  c[1] = (uint64_t)bx * blk + lx;
+  c[0] = lx > 0 ? c[1] - 1 : c[1] - blk + (g.BS0 - 1);
+  c[2] = lx + 1 < g.BS0 ? c[1] + 1 : c[1] + blk - (g.BS0 - 1);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int u0 = x + k % 3 - 1, u1 = y + k / 3 - 1;
+    inr[k] = u0 >= 0 && u1 >= 0 && u0 < g.DS0 && u1 < g.DS1;
+    en[k] = inr[k] ? r[k / 3] + c[k % 3] : r[1] + c[1];
+  }
 }
 
 // code of the neighbour at (i0, i1) from a cell when it flows into that cell
@@ -77,14 +106,14 @@ __device__ __forceinline__ bool outflow(unsigned code, int& dx, int& dy) {
   }
 }
 
-// cell (x, y) of element e of the block-tiled raster; false for block padding
+// cell (x, y) of element e of the block-tiled raster (e < 2^32); false for block padding
 __device__ __forceinline__ bool cell_of(const Geo& g, uint64_t e, int& x, int& y) {
-  const uint64_t bsz = (uint64_t)g.BS0 * g.BS1;
-  const int b = (int)(e / bsz);
-  const int r = (int)(e - (uint64_t)b * bsz);
-  const int by = b / g.NB0, bx = b - by * g.NB0;
-  x = bx * g.BS0 + r % g.BS0;
-  y = by * g.BS1 + r / g.BS0;
+  const unsigned bsz = (unsigned)g.BS0 * (unsigned)g.BS1;
+  const unsigned b = (unsigned)e / bsz, r = (unsigned)e - b * bsz;
+  const unsigned by = b / (unsigned)g.NB0, bx = b - by * (unsigned)g.NB0;
+  const unsigned ry = r / (unsigned)g.BS0;
+  x = (int)(bx * g.BS0 + (r - ry * g.BS0));
+  y = (int)(by * g.BS1 + ry);
   return x < g.DS0 && y < g.DS1;
 }
 
@@ -99,7 +128,7 @@ struct TileLds {
   unsigned left[TS * TS];      // in-tile upstream neighbours not yet computed
   uint8_t open[TS * TS];       // needs a total from outside the tile
   uint8_t done[TS * TS];
-  int any[2];                  // a round computed something (two slots: one is reset while the other is read)
+  uint64_t rt[TS + 2], ct[TS + 2];  // row and column terms of the tile and its ring
 };
 
 template <class T>
@@ -110,12 +139,31 @@ __global__ void __launch_bounds__(NT) k_tile(const T* __restrict__ in, const uin
   const int tx0 = (int)(blockIdx.x % (unsigned)ntx) * TS, ty0 = (int)(blockIdx.x / (unsigned)ntx) * TS;
   const int t = threadIdx.x;
   if (blockIdx.x == 0 && t == 0) queue[0] = queue[1] = 0;  // sources listed, sources taken
-  for (int i = t; i < (TS + 2) * (TS + 2); i += NT) {
-    const int lx = i % (TS + 2) - 1, ly = i / (TS + 2) - 1;
-    const int x = tx0 + lx, y = ty0 + ly;
-    L.dir[ly + 1][lx + 1] = (x >= 0 && y >= 0 && x < g.DS0 && y < g.DS1) ? dir[at(g, x, y)] : 0;
+  if (t < TS + 2) L.rt[t] = ty0 + t - 1 >= 0 && ty0 + t - 1 < g.DS1 ? rowt(g, ty0 + t - 1) : 0;
+  else if (t < 2 * (TS + 2)) L.ct[t - (TS + 2)] = tx0 + t - (TS + 2) - 1 >= 0 && tx0 + t - (TS + 2) - 1 < g.DS0 ? colt(g, tx0 + t - (TS + 2) - 1) : 0;
+  __syncthreads();
+  {
+    constexpr int RING = (TS + 2) * (TS + 2), PER = (RING + NT - 1) / NT;
+    uint8_t v[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {  // every load in flight before the first LDS write
+      const int i = t + j * NT, lx = i % (TS + 2), ly = i / (TS + 2);
+      const int x = tx0 + lx - 1, y = ty0 + ly - 1;
+      v[j] = (i < RING && x >= 0 && y >= 0 && x < g.DS0 && y < g.DS1) ? dir[L.rt[ly] + L.ct[lx]] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+      const int i = t + j * NT;
+      if (i < RING) L.dir[i / (TS + 2)][i % (TS + 2)] = v[j];
+    }
   }
   __syncthreads();
+  T own[CPT];
+#pragma unroll
+  for (int j = 0; j < CPT; j++) {  // every load in flight before the first is used
+    const int i = t + j * NT, lx = i % TS, ly = i / TS;
+    own[j] = tx0 + lx < g.DS0 && ty0 + ly < g.DS1 ? in[L.rt[ly + 1] + L.ct[lx + 1]] : T(0);
+  }
 #pragma unroll
   for (int j = 0; j < CPT; j++) {
     const int i = t + j * NT, lx = i % TS, ly = i / TS;
@@ -132,49 +180,47 @@ __global__ void __launch_bounds__(NT) k_tile(const T* __restrict__ in, const uin
     L.left[i] = kin;
     L.open[i] = (uint8_t)ext;
     L.done[i] = !inr;
-    L.tot[i] = inr ? in[at(g, x, y)] : T(0);
+    L.tot[i] = own[j];
   }
-  if (t == 0) L.any[0] = L.any[1] = 0;
   __syncthreads();
-  // rounds: the cells whose in-tile upstream cells are all done; a cell with an open
-  // upstream cell is open, any other gets in + its upstream totals in the fixed order
-  for (int round = 0;; round++) {
-    unsigned ready = 0;
+  // the tile's sources (no upstream cell inside the tile), fixed before any count falls
+  unsigned srcs = 0;
 #pragma unroll
-    for (int j = 0; j < CPT; j++) {
-      const int i = t + j * NT;
-      if (!L.done[i] && L.left[i] == 0) ready |= 1u << j;
-    }
-    __syncthreads();  // the round's set is fixed before any of it changes
-    if (ready) {
-#pragma unroll
-      for (int j = 0; j < CPT; j++) {
-        if (!(ready >> j & 1u)) continue;
-        const int i = t + j * NT, lx = i % TS, ly = i / TS;
-        const bool op = L.open[i];
-        if (!op) {
-          T acc = L.tot[i];
-          for (int i1 = -1; i1 <= 1; i1++)
-            for (int i0 = -1; i0 <= 1; i0++)
-              if ((i0 | i1) != 0 && L.dir[ly + 1 + i1][lx + 1 + i0] == inflow_code(i0, i1))
-                acc = (T)(acc + L.tot[(ly + i1) * TS + lx + i0]);  // inside the tile: the cell is closed
-          L.tot[i] = acc;
-        }
-        L.done[i] = 1;
-        int dx, dy;
-        if (outflow(L.dir[ly + 1][lx + 1], dx, dy) && lx + dx >= 0 && lx + dx < TS && ly + dy >= 0 && ly + dy < TS) {
-          const int d = (ly + dy) * TS + lx + dx;
-          if (op) L.open[d] = 1;
-          __hip_atomic_fetch_add(&L.left[d], 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-      }
-      L.any[round & 1] = 1;
-    }
-    __syncthreads();
-    const bool more = L.any[round & 1];
-    if (t == 0) L.any[(round + 1) & 1] = 0;  // last read before the previous round's second barrier
-    if (!more) break;
+  for (int j = 0; j < CPT; j++) {
+    const int i = t + j * NT;
+    if (!L.done[i] && L.left[i] == 0) srcs |= 1u << j;
   }
+  __syncthreads();
+  // walks in LDS from each source, with the hand-off of the global walks: a cell with
+  // an open upstream cell is open, any other gets in + its upstream totals in the
+  // fixed order; the walk that takes the downstream count to zero goes on
+#pragma unroll 1
+  for (int j = 0; j < CPT; j++) {
+    if (!(srcs >> j & 1u)) continue;
+    int i = t + j * NT;
+    for (;;) {
+      const int lx = i % TS, ly = i / TS;
+      const bool op = L.open[i];
+      if (!op) {
+        T acc = L.tot[i];
+        for (int i1 = -1; i1 <= 1; i1++)
+          for (int i0 = -1; i0 <= 1; i0++)
+            if ((i0 | i1) != 0 && L.dir[ly + 1 + i1][lx + 1 + i0] == inflow_code(i0, i1))
+              acc = (T)(acc + L.tot[(ly + i1) * TS + lx + i0]);  // inside the tile: the cell is closed
+        L.tot[i] = acc;
+      }
+      L.done[i] = 1;
+      int dx, dy;
+      if (!outflow(L.dir[ly + 1][lx + 1], dx, dy) || lx + dx < 0 || lx + dx >= TS || ly + dy < 0 || ly + dy >= TS) break;
+      const int d = (ly + dy) * TS + lx + dx;
+      if (op) L.open[d] = 1;
+      // release / acquire at work-group scope: the total and the open mark are in LDS
+      // before the count falls, and the walk that takes it to zero reads them after
+      if (__hip_atomic_fetch_add(&L.left[d], 0xffffffffu, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) != 1u) break;
+      i = d;
+    }
+  }
+  __syncthreads();
   // closed cells are final; open (and cycle) cells start at 0 for the walks
 #pragma unroll
   for (int j = 0; j < CPT; j++) {
@@ -185,23 +231,30 @@ __global__ void __launch_bounds__(NT) k_tile(const T* __restrict__ in, const uin
     int dx, dy;
     const bool down = outflow(L.dir[ly + 1][lx + 1], dx, dy) && x + dx >= 0 && x + dx < g.DS0 && y + dy >= 0 &&
                       y + dy < g.DS1;
-    const uint64_t e = at(g, x, y);
+    const uint64_t e = L.rt[ly + 1] + L.ct[lx + 1];
     out[e] = closed && down ? L.tot[i] : T(0);
     cnt[e] = closed ? CLOSED : 0u;
   }
 }
 
-// Upstream neighbours of (x, y) that are open: their word is not CLOSED (it is 0,
-// open and not yet counted, or a count, never CLOSED).
-__device__ __forceinline__ unsigned open_upstream(const uint8_t* dir, unsigned* cnt, const Geo& g, int x, int y) {
+// Upstream neighbours of (x, y) that are open: their word is not CLOSED. k_tile wrote
+// every word; here an open cell's word goes from 0 to its count, neither of which is
+// CLOSED, so a plain load of either value gives the same answer. All 18 loads are
+// issued before the first is used.
+__device__ __forceinline__ unsigned open_upstream(const uint8_t* dir, const unsigned* cnt, const Geo& g, int x, int y) {
+  uint64_t en[9];
+  bool inr[9];
+  hood(g, x, y, en, inr);
+  unsigned dn[9], st[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    dn[k] = dir[en[k]];
+    st[k] = cnt[en[k]];
+  }
   unsigned c = 0;
-  for (int i1 = -1; i1 <= 1; i1++)
-    for (int i0 = -1; i0 <= 1; i0++) {
-      const int u0 = x + i0, u1 = y + i1;
-      if ((i0 | i1) == 0 || u0 < 0 || u1 < 0 || u0 >= g.DS0 || u1 >= g.DS1) continue;
-      const uint64_t eu = at(g, u0, u1);
-      c += dir[eu] == inflow_code(i0, i1) && __hip_atomic_load(&cnt[eu], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != CLOSED;
-    }
+#pragma unroll
+  for (int k = 0; k < 9; k++)
+    c += k != 4 && inr[k] && dn[k] == inflow_code(k % 3 - 1, k / 3 - 1) && st[k] != CLOSED;
   return c;
 }
 
@@ -217,18 +270,24 @@ __global__ void __launch_bounds__(NT) k_count(const uint8_t* __restrict__ dir, u
       cnt[e] = CLOSED;
     } else if (cnt[e] != CLOSED) {
       const unsigned c = open_upstream(dir, cnt, g, x, y);
-      __hip_atomic_store(&cnt[e], c == 0 ? SRC : c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      cnt[e] = c == 0 ? SRC : c;
       src = c == 0;
     }
   }
-  // one atomic per wavefront reserves the slots of its sources
+  // slots for the sources: an LDS count per work-group, then one global atomic per
+  // work-group (one address shared by the whole grid would serialise otherwise)
+  __shared__ unsigned wg_n, wg_base;
+  if (threadIdx.x == 0) wg_n = 0;
+  __syncthreads();
   const unsigned long long m = __ballot(src);
-  if (m == 0) return;
   const int lane = __lane_id();
-  unsigned base = 0;
-  if (lane == __ffsll((long long)m) - 1) base = atomicAdd(&queue[0], (unsigned)__popcll(m));
-  base = __shfl(base, __ffsll((long long)m) - 1, 64);
-  if (src) queue[2 + base + __popcll(m & ((1ull << lane) - 1))] = (unsigned)e;
+  unsigned wave_base = 0;
+  if (m != 0 && lane == __ffsll((long long)m) - 1) wave_base = atomicAdd(&wg_n, (unsigned)__popcll(m));
+  wave_base = __shfl(wave_base, m != 0 ? __ffsll((long long)m) - 1 : 0, 64);
+  __syncthreads();
+  if (threadIdx.x == 0) wg_base = wg_n ? atomicAdd(&queue[0], wg_n) : 0;
+  __syncthreads();
+  if (src) queue[2 + wg_base + wave_base + __popcll(m & ((1ull << lane) - 1))] = (unsigned)e;
 }
 
 
@@ -259,12 +318,7 @@ __device__ void walk(const T* __restrict__ in, const uint8_t* __restrict__ dir, 
     // a neighbour outside it never matches), then one round for the upstream totals
     uint64_t en[9];
     bool inr[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-      const int u0 = x + k % 3 - 1, u1 = y + k / 3 - 1;
-      inr[k] = u0 >= 0 && u1 >= 0 && u0 < g.DS0 && u1 < g.DS1;
-      en[k] = inr[k] ? at(g, u0, u1) : e;
-    }
+    hood(g, x, y, en, inr);
     unsigned dn[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) dn[k] = dir[en[k]];
@@ -285,7 +339,10 @@ __device__ void walk(const T* __restrict__ in, const uint8_t* __restrict__ dir, 
     y += dy;
     if (x < 0 || y < 0 || x >= g.DS0 || y >= g.DS1) return;  // flows off the raster
     st_total(&out[e], acc);
-    const uint64_t ed = at(g, x, y);
+    uint64_t ed = en[4];
+#pragma unroll
+    for (int k = 0; k < 9; k++)  // a select per slot: a computed index would put en[] in scratch memory
+      if (k == (dy + 1) * 3 + dx + 1) ed = en[k];
     if (__hip_atomic_load(&cnt[ed], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 1u) {
       // the total has reached the coherence point before the count falls, so the walk
       // that takes the count to zero reads it (its loads are issued after the atomic returns)

@@ -43,11 +43,14 @@ def emulate(v, d, ts, seed):
                             else:
                                 ext = True
                 left[(x, y)], opn[(x, y)], done[(x, y)], tot[(x, y)] = kin, ext, False, T(v[y, x])
-            while True:
-                ready = [c for c in cells if not done[c] and left[c] == 0]
-                if not ready:
-                    break
-                for (x, y) in ready:
+            # walks from the tile's sources (fixed before any count falls), in a shuffled order;
+            # the walk that takes a downstream count to zero goes on
+            tsrcs = [c for c in cells if left[c] == 0]
+            np.random.default_rng(seed + tx0 + 7 * ty0).shuffle(tsrcs)
+            for c in tsrcs:
+                x, y = c
+                while True:
+                    assert not done[(x, y)]
                     if not opn[(x, y)]:
                         acc = tot[(x, y)]
                         for i1 in (-1, 0, 1):
@@ -59,9 +62,13 @@ def emulate(v, d, ts, seed):
                         tot[(x, y)] = acc
                     done[(x, y)] = True
                     dn = down(x, y)
-                    if dn is not None and inside(*dn):
-                        opn[dn] = opn[dn] or opn[(x, y)]
-                        left[dn] -= 1
+                    if dn is None or not inside(*dn):
+                        break
+                    opn[dn] = opn[dn] or opn[(x, y)]
+                    left[dn] -= 1
+                    if left[dn] != 0:
+                        break
+                    x, y = dn
             for c in cells:
                 closed = done[c] and not opn[c]
                 out[c[1], c[0]] = tot[c] if closed and down(*c) is not None else T(0)
@@ -112,3 +119,45 @@ def test_three_passes_match_oracle(O, ts, seed):
             got = emulate(v, d, ts, seed)
             want = O.spread_scan(O.from_array(v, (16, 16)), O.from_array(d, (16, 16))).val
             assert np.array_equal(got, want), (ts, v.dtype)
+
+
+def _hood(BS0, BS1, NB0, DS0, DS1, x, y):
+    """spread.hip hood(): neighbour element indices from one division pair per axis and carries."""
+    by, bx = y // BS1, x // BS0
+    ly, lx = y - by * BS1, x - bx * BS0
+    blk, row = BS1 * BS0, NB0 * BS1 * BS0
+    r1 = by * row + ly * BS0
+    r = [r1 - BS0 if ly > 0 else r1 - row + (BS1 - 1) * BS0, r1,
+         r1 + BS0 if ly + 1 < BS1 else r1 + row - (BS1 - 1) * BS0]
+    c1 = bx * blk + lx
+    c = [c1 - 1 if lx > 0 else c1 - blk + (BS0 - 1), c1, c1 + 1 if lx + 1 < BS0 else c1 + blk - (BS0 - 1)]
+    out = []
+    for k in range(9):
+        u0, u1 = x + k % 3 - 1, y + k // 3 - 1
+        inr = 0 <= u0 < DS0 and 0 <= u1 < DS1
+        out.append((inr, r[k // 3] + c[k % 3] if inr else r1 + c1))
+    return out
+
+
+@pytest.mark.parametrize("ds,bs", [((70, 50), (16, 8)), ((33, 47), (32, 16)), ((64, 64), (64, 64)), ((9, 5), (4, 2))])
+def test_hood_indices_are_block_tiled_indices(ds, bs):
+    """Every neighbour index hood() forms is the block-tiled index of that cell, and cell_of()
+    inverts the index of every cell (the padding of edge blocks is never reached)."""
+    DS0, DS1 = ds
+    BS0, BS1 = bs
+    NB0, NB1 = -(-DS0 // BS0), -(-DS1 // BS1)
+
+    def at(x, y):
+        bx, by = x // BS0, y // BS1
+        return ((by * NB0 + bx) * BS1 + (y - by * BS1)) * BS0 + (x - bx * BS0)
+
+    n = NB0 * NB1 * BS0 * BS1
+    for y in range(DS1):
+        for x in range(DS0):
+            e = at(x, y)
+            b, r = e // (BS0 * BS1), e % (BS0 * BS1)
+            assert ((b % NB0) * BS0 + r % BS0, (b // NB0) * BS1 + r // BS0) == (x, y)
+            for k, (inr, idx) in enumerate(_hood(BS0, BS1, NB0, DS0, DS1, x, y)):
+                u0, u1 = x + k % 3 - 1, y + k // 3 - 1
+                assert 0 <= idx < n
+                assert idx == (at(u0, u1) if inr else e)
