@@ -551,3 +551,35 @@ def test_stats_barrier_access_lhs_access(G, O):
     oc[[70, 40]] = 7.25
     oc[[0, 89]] = -1
     assert_exact(G.to_array(C), oc.val, "lhs access")
+
+
+def test_hillshade_4096_error_per_output(G, O):
+    """The plain hillshade workload at its benchmark size (4096x4096 F32 DEM of seeded Gaussians
+    + fBm, 512x512 blocks): slope, aspect and hillshade from the GPU against the FP64 oracle, and
+    the FP32-semantics oracle against the same FP64 values, reported per output. The GPU may not
+    be further from FP64 than twice the FP32 evaluation of the script itself is."""
+    from map_amd import workloads as W
+    print("case hillshade 4096x4096 error per output", flush=True)
+    side, bs = 4096, (512, 512)
+    rng = np.random.Generator(np.random.PCG64(0))
+    D = G.gen_dem((side, side), bs, W.dem_gaussians(rng, side, side), 0)
+    dem = G.to_array(D)
+    outs = {"slope": W.slope, "aspect": W.aspect, "hillshade": W.hillshade}
+    got = {k: G.to_array(f(G, D)).astype(np.float64) for k, f in outs.items()}
+    ref32 = {k: f(O, O.from_array(dem, bs)).val.astype(np.float64) for k, f in outs.items()}
+    O.set_mode("f64")
+    od = O.from_array(dem, bs)
+    ref64 = {k: np.asarray(f(O, od).val, np.float64) for k, f in outs.items()}
+    t1, t2 = W.hillshade_terms(O, od)
+    O.set_mode("f32")
+    scale = {"slope": None, "aspect": None, "hillshade": np.abs(t1.val) + np.abs(t2.val)}
+    for k in outs:
+        rg = fp32_report(got[k], ref64[k], scale[k])
+        ro = fp32_report(ref32[k], ref64[k], scale[k])
+        d32 = np.abs(got[k] - ref32[k])
+        print("  %-9s gpu vs fp64: max %.3g abs, %.3g ulp, %d cells > 4 ulp | fp32 oracle vs fp64: max %.3g abs, "
+              "%.3g ulp, %d cells > 4 ulp | gpu vs fp32 oracle: max %.3g abs"
+              % (k, rg["max_abs"], rg["max_ulp"], rg["n_over_4ulp"], ro["max_abs"], ro["max_ulp"], ro["n_over_4ulp"],
+                 d32.max()), flush=True)
+        assert np.all(np.isfinite(got[k]))
+        assert rg["max_abs"] <= 2 * ro["max_abs"] + 1e-6, k
