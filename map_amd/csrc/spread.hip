@@ -121,6 +121,14 @@ constexpr unsigned CLOSED = 0x200u;  // status word of a cell computed by k_tile
 constexpr int TS = 64;               // tile side of k_tile
 constexpr int CPT = TS * TS / NT;    // cells per thread in k_tile
 
+// Cell j of thread t in a tile: row ly = t/64 + 4j, column lx = (t - ly) mod 64. Every row
+// and every column of the tile meets 64 different threads, so the sources along a tile
+// edge (where flow enters) start their walks in parallel whatever the flow direction.
+__device__ __forceinline__ int tile_cell(int t, int j) {
+  const int ly = t / TS + (NT / TS) * j;
+  return ly * TS + ((t % TS - ly) & (TS - 1));
+}
+
 template <class T>
 struct TileLds {
   T tot[TS * TS];              // totals of the tile's cells (input values until computed)
@@ -161,12 +169,12 @@ __global__ void __launch_bounds__(NT) k_tile(const T* __restrict__ in, const uin
   T own[CPT];
 #pragma unroll
   for (int j = 0; j < CPT; j++) {  // every load in flight before the first is used
-    const int i = t + j * NT, lx = i % TS, ly = i / TS;
+    const int i = tile_cell(t, j), lx = i % TS, ly = i / TS;
     own[j] = tx0 + lx < g.DS0 && ty0 + ly < g.DS1 ? in[L.rt[ly + 1] + L.ct[lx + 1]] : T(0);
   }
 #pragma unroll
   for (int j = 0; j < CPT; j++) {
-    const int i = t + j * NT, lx = i % TS, ly = i / TS;
+    const int i = tile_cell(t, j), lx = i % TS, ly = i / TS;
     const int x = tx0 + lx, y = ty0 + ly;
     const bool inr = x < g.DS0 && y < g.DS1;
     unsigned kin = 0, ext = 0;
@@ -187,7 +195,7 @@ __global__ void __launch_bounds__(NT) k_tile(const T* __restrict__ in, const uin
   unsigned srcs = 0;
 #pragma unroll
   for (int j = 0; j < CPT; j++) {
-    const int i = t + j * NT;
+    const int i = tile_cell(t, j);
     if (!L.done[i] && L.left[i] == 0) srcs |= 1u << j;
   }
   __syncthreads();
@@ -197,7 +205,7 @@ __global__ void __launch_bounds__(NT) k_tile(const T* __restrict__ in, const uin
 #pragma unroll 1
   for (int j = 0; j < CPT; j++) {
     if (!(srcs >> j & 1u)) continue;
-    int i = t + j * NT;
+    int i = tile_cell(t, j);
     for (;;) {
       const int lx = i % TS, ly = i / TS;
       const bool op = L.open[i];
@@ -224,7 +232,7 @@ __global__ void __launch_bounds__(NT) k_tile(const T* __restrict__ in, const uin
   // closed cells are final; open (and cycle) cells start at 0 for the walks
 #pragma unroll
   for (int j = 0; j < CPT; j++) {
-    const int i = t + j * NT, lx = i % TS, ly = i / TS;
+    const int i = tile_cell(t, j), lx = i % TS, ly = i / TS;
     const int x = tx0 + lx, y = ty0 + ly;
     if (x >= g.DS0 || y >= g.DS1) continue;
     const bool closed = L.done[i] && !L.open[i];

@@ -161,3 +161,20 @@ def test_hood_indices_are_block_tiled_indices(ds, bs):
                 u0, u1 = x + k % 3 - 1, y + k // 3 - 1
                 assert 0 <= idx < n
                 assert idx == (at(u0, u1) if inr else e)
+
+
+def test_tile_cell_mapping_spreads_rows_and_columns():
+    """spread.hip tile_cell(): the 4096 cells of a 64x64 tile go to 256 threads, 16 each, and
+    every row and every column of the tile meets 64 different threads."""
+    TS, NT = 64, 256
+    owner = {}
+    for t in range(NT):
+        for j in range(TS * TS // NT):
+            ly = t // TS + (NT // TS) * j
+            i = ly * TS + ((t % TS - ly) & (TS - 1))
+            assert i not in owner
+            owner[i] = t
+    assert len(owner) == TS * TS
+    for r in range(TS):
+        assert len({owner[r * TS + c] for c in range(TS)}) == TS
+        assert len({owner[c * TS + r] for c in range(TS)}) == TS
