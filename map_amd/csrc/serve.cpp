@@ -81,7 +81,7 @@ long long Runtime::serveSubmit(const std::vector<Node*>& targets) {
   auto r = std::make_unique<Request>();
   r->id = srv_->next_id++;
   r->ev.targets = with_aux(targets);
-  for (Node* t : targets) t->ref++;  // held until the request retires
+  for (Node* t : r->ev.targets) t->ref++;  // held until the request retires (released in serve_retire)
   r->t_submit = now_s();
   srv_->queued.push_back(std::move(r));
   return srv_->next_id - 1;

@@ -583,3 +583,56 @@ def test_hillshade_4096_error_per_output(G, O):
                  d32.max()), flush=True)
         assert np.all(np.isfinite(got[k]))
         assert rg["max_abs"] <= 2 * ro["max_abs"] + 1e-6, k
+
+
+def test_serve_mix_with_scans_and_cell_reads(G, O):
+    """Served requests that run precompiled scans (flow accumulation, radial max), cell reads
+    (a D0 value from a raster) and Stats beside generated-kernel requests, in waves under a
+    small HBM budget: every result equals the same request evaluated alone, and the scans equal
+    the oracle."""
+    bs = (128, 128)
+    sizes = [(300, 200), (257, 389), (500, 310)]
+    dem_np = [dem_array(w, h, 60 + i) for i, (w, h) in enumerate(sizes)]
+    dir_np = [scripts.d8_directions(d.astype(np.float64)) for d in dem_np]
+
+    def request(kind, di, D, R):
+        if kind == "flow":
+            return (G.flow_accumulation(G.ones_like(D) * 1.0, R),)
+        if kind == "rmax":
+            return (G.rmax(D * 1.0, [sizes[di][0] // 3, sizes[di][1] // 2]),)
+        if kind == "cell":
+            return ((D * 2.0)[[sizes[di][0] - 1, sizes[di][1] // 2]],)
+        if kind == "stats":
+            return (G.stats(D + 1.0),)
+        return (scripts.hillshade(G, D),)
+
+    def results(outs):
+        return [G.value(o) if o.numdim() == G.D0 else G.to_array(o) for o in outs]
+
+    rng = np.random.Generator(np.random.PCG64(61))
+    kinds = ["flow", "rmax", "cell", "stats", "hill"]
+    reqs = [(kinds[int(rng.integers(5))], int(rng.integers(len(sizes)))) for _ in range(15)]
+    print("case serve with scans", reqs, flush=True)
+    dems = [G.from_array(d, bs) for d in dem_np]
+    dirs = [G.from_array(d, bs) for d in dir_np]
+    srv = G.Server(hbm_budget=4 << 20, max_inflight=4)
+    got, ids = {}, []
+    try:
+        for wave in range(0, len(reqs), 4):
+            for kind, di in reqs[wave:wave + 4]:
+                ids.append(srv.submit(*request(kind, di, dems[di], dirs[di])))
+            for rid, *_ in srv.step(0):
+                got[rid] = results(srv.result(rid))
+        while len(got) < len(ids):
+            for rid, *_ in srv.step(2000):
+                got[rid] = results(srv.result(rid))
+    finally:
+        srv.close()
+    for i, (kind, di) in enumerate(reqs):
+        outs = request(kind, di, G.from_array(dem_np[di], bs), G.from_array(dir_np[di], bs))
+        G.eval(*outs)
+        for a, b in zip(got[ids[i]], results(outs)):
+            assert_exact(np.asarray(a), np.asarray(b), "request %d (%s)" % (i, kind))
+        if kind == "flow":
+            want = O.flow_accumulation(O.from_array(np.ones_like(dem_np[di]), bs), O.from_array(dir_np[di], bs)).val
+            assert_exact(got[ids[i]][0], want, "flow vs oracle")
