@@ -1190,8 +1190,9 @@ void Runtime::evaluate(const std::vector<Node*>& targets) {
   cnt.ms_last_eval = t.ms();
 }
 
-std::string Runtime::plan(const std::vector<Node*>& targets) {
+std::string Runtime::plan(const std::vector<Node*>& targets_in) {
   std::lock_guard<std::recursive_mutex> lk(mtx_);
+  const std::vector<Node*> targets = with_aux(targets_in);  // as an evaluation would run them
   std::vector<Node*> list = cut_list(targets);
   struct Uncut {
     std::vector<Node*>& l;
