@@ -14,7 +14,7 @@
 // The reference iterates over the whole raster until it is stable (one
 // step per cell of the longest flow path). Here every cell is computed once,
 // in three passes:
-//   k_tile: a work-group per 64x64 tile computes, in LDS, every cell whose
+//   k_tile: a work-group per 32x32 tile computes, in LDS, every cell whose
 //     upstream cells all lie in the tile ("closed" cells: their totals need
 //     nothing from outside), by walks from the tile's sources with the
 //     hand-off described below, and marks the others "open" (a cell with an
@@ -118,11 +118,11 @@ __device__ __forceinline__ bool cell_of(const Geo& g, uint64_t e, int& x, int& y
 }
 
 constexpr unsigned CLOSED = 0x200u;  // status word of a cell computed by k_tile (or block padding)
-constexpr int TS = 64;               // tile side of k_tile
+constexpr int TS = 32;               // tile side of k_tile (46 KB of LDS at 64 left more CUs idle than busy)
 constexpr int CPT = TS * TS / NT;    // cells per thread in k_tile
 
-// Cell j of thread t in a tile: row ly = t/64 + 4j, column lx = (t - ly) mod 64. Every row
-// and every column of the tile meets 64 different threads, so the sources along a tile
+// Cell j of thread t in a tile: row ly = t/TS + (NT/TS)j, column lx = (t - ly) mod TS. Every
+// row and every column of the tile meets TS different threads, so the sources along a tile
 // edge (where flow enters) start their walks in parallel whatever the flow direction.
 __device__ __forceinline__ int tile_cell(int t, int j) {
   const int ly = t / TS + (NT / TS) * j;

@@ -163,10 +163,11 @@ def test_hood_indices_are_block_tiled_indices(ds, bs):
                 assert idx == (at(u0, u1) if inr else e)
 
 
-def test_tile_cell_mapping_spreads_rows_and_columns():
-    """spread.hip tile_cell(): the 4096 cells of a 64x64 tile go to 256 threads, 16 each, and
-    every row and every column of the tile meets 64 different threads."""
-    TS, NT = 64, 256
+@pytest.mark.parametrize("TS", [32, 64, 16])
+def test_tile_cell_mapping_spreads_rows_and_columns(TS):
+    """spread.hip tile_cell(): the TS*TS cells of a tile go to 256 threads in equal shares,
+    and every row and every column of the tile meets TS different threads."""
+    NT = 256
     owner = {}
     for t in range(NT):
         for j in range(TS * TS // NT):
