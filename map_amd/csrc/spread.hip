@@ -13,20 +13,20 @@
 //
 // The reference iterates over the whole raster until it is stable (one
 // step per cell of the longest flow path). Here every cell is computed once,
-// in three passes:
+// in two passes:
 //   k_tile: a work-group per 32x32 tile computes, in LDS, every cell whose
 //     upstream cells all lie in the tile ("closed" cells: their totals need
 //     nothing from outside), by walks from the tile's sources with the
 //     hand-off described below, and marks the others "open" (a cell with an
 //     upstream neighbour outside the tile, or downstream of an open cell, or
-//     on a direction cycle);
-//   k_count: every open cell counts its open upstream neighbours and lists the
-//     sources (open cells with none);
-//   k_walk: a persistent grid walks from the listed sources. A walk computes T(c) from the finished
-//     upstream totals, stores it, and decrements the count of c's downstream
-//     cell; the walk that takes the count to zero has seen every other
-//     upstream total of that cell (each was stored before its own decrement)
-//     and continues there; the others end.
+//     on a direction cycle); it assembles each open cell's count of open
+//     upstream neighbours with atomics and lists the open cells whose count
+//     is zero (the walk sources);
+//   k_walk: a persistent grid walks from the listed sources. A walk computes
+//     T(c) from the finished upstream totals, stores it, and decrements the
+//     count of c's downstream cell; the walk that takes the count to zero has
+//     seen every other upstream total of that cell (each was stored before
+//     its own decrement) and continues there; the others end.
 // Each cell is computed exactly once, every walk ends (each step takes a
 // counter that only falls), and every sum follows the fixed neighbour order,
 // so the result does not depend on the schedule. Cells on a direction cycle
@@ -44,7 +44,6 @@ namespace mapx {
 namespace spread {
 
 constexpr int NT = 256;
-constexpr unsigned SRC = 0x100u;  // count word of a source cell (no upstream neighbour); never decremented
 
 struct Geo {
   int BS0, BS1, NB0, NB1, DS0, DS1;
@@ -117,7 +116,6 @@ __device__ __forceinline__ bool cell_of(const Geo& g, uint64_t e, int& x, int& y
   return x < g.DS0 && y < g.DS1;
 }
 
-constexpr unsigned CLOSED = 0x200u;  // status word of a cell computed by k_tile (or block padding)
 constexpr int TS = 32;               // tile side of k_tile (46 KB of LDS at 64 left more CUs idle than busy)
 constexpr int CPT = TS * TS / NT;    // cells per thread in k_tile
 
@@ -137,6 +135,8 @@ struct TileLds {
   uint8_t open[TS * TS];       // needs a total from outside the tile
   uint8_t done[TS * TS];
   uint64_t rt[TS + 2], ct[TS + 2];  // row and column terms of the tile and its ring
+  unsigned src[TS * TS];       // walk sources found by this work-group
+  unsigned nsrc, base;
 };
 
 template <class T>
@@ -146,7 +146,6 @@ __global__ void __launch_bounds__(NT) k_tile(const T* __restrict__ in, const uin
   __shared__ TileLds<T> L;
   const int tx0 = (int)(blockIdx.x % (unsigned)ntx) * TS, ty0 = (int)(blockIdx.x / (unsigned)ntx) * TS;
   const int t = threadIdx.x;
-  if (blockIdx.x == 0 && t == 0) queue[0] = queue[1] = 0;  // sources listed, sources taken
   if (t < TS + 2) L.rt[t] = ty0 + t - 1 >= 0 && ty0 + t - 1 < g.DS1 ? rowt(g, ty0 + t - 1) : 0;
   else if (t < 2 * (TS + 2)) L.ct[t - (TS + 2)] = tx0 + t - (TS + 2) - 1 >= 0 && tx0 + t - (TS + 2) - 1 < g.DS0 ? colt(g, tx0 + t - (TS + 2) - 1) : 0;
   __syncthreads();
@@ -229,7 +228,15 @@ __global__ void __launch_bounds__(NT) k_tile(const T* __restrict__ in, const uin
     }
   }
   __syncthreads();
-  // closed cells are final; open (and cycle) cells start at 0 for the walks
+  // closed cells are final; open (and cycle) cells start at 0 for the walks. The count of
+  // an open cell (its open upstream neighbours) is assembled in `cnt` (cleared before) from
+  // two sides, in any order: its own tile adds its open in-tile upstream cells plus all its
+  // upstream cells outside the tile, and the tile of each such cell that is closed takes one
+  // back. Every step before the last leaves a non-zero value (negative before the addition,
+  // above the final count after it), so the atomic that leaves zero is the last one, and it
+  // lists the cell as a source of the walks.
+  if (t == 0) L.nsrc = 0;
+  __syncthreads();
 #pragma unroll
   for (int j = 0; j < CPT; j++) {
     const int i = tile_cell(t, j), lx = i % TS, ly = i / TS;
@@ -241,63 +248,35 @@ __global__ void __launch_bounds__(NT) k_tile(const T* __restrict__ in, const uin
                       y + dy < g.DS1;
     const uint64_t e = L.rt[ly + 1] + L.ct[lx + 1];
     out[e] = closed && down ? L.tot[i] : T(0);
-    cnt[e] = closed ? CLOSED : 0u;
-  }
-}
-
-// Upstream neighbours of (x, y) that are open: their word is not CLOSED. k_tile wrote
-// every word; here an open cell's word goes from 0 to its count, neither of which is
-// CLOSED, so a plain load of either value gives the same answer. All 18 loads are
-// issued before the first is used.
-__device__ __forceinline__ unsigned open_upstream(const uint8_t* dir, const unsigned* cnt, const Geo& g, int x, int y) {
-  uint64_t en[9];
-  bool inr[9];
-  hood(g, x, y, en, inr);
-  unsigned dn[9], st[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) {
-    dn[k] = dir[en[k]];
-    st[k] = cnt[en[k]];
-  }
-  unsigned c = 0;
-#pragma unroll
-  for (int k = 0; k < 9; k++)
-    c += k != 4 && inr[k] && dn[k] == inflow_code(k % 3 - 1, k / 3 - 1) && st[k] != CLOSED;
-  return c;
-}
-
-// Open cells count their open upstream neighbours (a word that is not CLOSED) and the
-// sources among them (none) are listed in queue[2..]; block padding becomes CLOSED.
-__global__ void __launch_bounds__(NT) k_count(const uint8_t* __restrict__ dir, unsigned* cnt, unsigned* queue, Geo g,
-                                              uint64_t n) {
-  const uint64_t e = (uint64_t)blockIdx.x * NT + threadIdx.x;
-  bool src = false;
-  if (e < n) {
-    int x, y;
-    if (!cell_of(g, e, x, y)) {
-      cnt[e] = CLOSED;
-    } else if (cnt[e] != CLOSED) {
-      const unsigned c = open_upstream(dir, cnt, g, x, y);
-      cnt[e] = c == 0 ? SRC : c;
-      src = c == 0;
+    if (closed) {
+      if (down && (lx + dx < 0 || lx + dx >= TS || ly + dy < 0 || ly + dy >= TS)) {
+        const uint64_t ed = L.rt[ly + 1 + dy] + L.ct[lx + 1 + dx];
+        if (atomicAdd(&cnt[ed], 0xffffffffu) == 1u) L.src[atomicAdd(&L.nsrc, 1u)] = (unsigned)ed;
+      }
+    } else {
+      unsigned k = 0;
+      for (int i1 = -1; i1 <= 1; i1++)
+        for (int i0 = -1; i0 <= 1; i0++) {
+          if ((i0 | i1) == 0 || L.dir[ly + 1 + i1][lx + 1 + i0] != inflow_code(i0, i1)) continue;
+          const int u = (ly + i1) * TS + lx + i0;
+          const bool inside = lx + i0 >= 0 && lx + i0 < TS && ly + i1 >= 0 && ly + i1 < TS;
+          k += !inside || !L.done[u] || L.open[u];
+        }
+      if (atomicAdd(&cnt[e], k) + k == 0u) L.src[atomicAdd(&L.nsrc, 1u)] = (unsigned)e;
     }
   }
-  // slots for the sources: an LDS count per work-group, then one global atomic per
-  // work-group (one address shared by the whole grid would serialise otherwise)
-  __shared__ unsigned wg_n, wg_base;
-  if (threadIdx.x == 0) wg_n = 0;
   __syncthreads();
-  const unsigned long long m = __ballot(src);
-  const int lane = __lane_id();
-  unsigned wave_base = 0;
-  if (m != 0 && lane == __ffsll((long long)m) - 1) wave_base = atomicAdd(&wg_n, (unsigned)__popcll(m));
-  wave_base = __shfl(wave_base, m != 0 ? __ffsll((long long)m) - 1 : 0, 64);
+  if (t == 0) L.base = L.nsrc ? atomicAdd(&queue[0], L.nsrc) : 0u;
   __syncthreads();
-  if (threadIdx.x == 0) wg_base = wg_n ? atomicAdd(&queue[0], wg_n) : 0;
-  __syncthreads();
-  if (src) queue[2 + wg_base + wave_base + __popcll(m & ((1ull << lane) - 1))] = (unsigned)e;
+  for (unsigned q = t; q < L.nsrc; q += NT) queue[2 + L.base + q] = L.src[q];
 }
 
+// Clears the count words and the two list counters before the tile pass.
+__global__ void __launch_bounds__(NT) k_clear(unsigned* cnt, unsigned* queue, uint64_t n) {
+  const uint64_t e = (uint64_t)blockIdx.x * NT + threadIdx.x;
+  if (e < n) cnt[e] = 0u;
+  if (e < 2) queue[e] = 0u;  // sources listed, sources taken
+}
 
 // Totals move between walks through the device's coherence point: agent-scope
 // loads and stores (they bypass the XCD-local L2 caches), ordered by waiting for the
@@ -385,13 +364,13 @@ static void check(hipError_t e, const char* w) {
 template <class T>
 static void launch(void* out, const void* in, const void* dir, void* cnt, const Geo& g, uint64_t n, hipStream_t s) {
   unsigned* queue = (unsigned*)cnt + n;  // 2 counters, then up to n listed sources
+  const unsigned grid = (unsigned)((n + NT - 1) / NT);
+  hipLaunchKernelGGL(k_clear, dim3(grid), dim3(NT), 0, s, (unsigned*)cnt, queue, n);
+  check(hipGetLastError(), "spread clear launch");
   const int ntx = (g.DS0 + TS - 1) / TS, nty = (g.DS1 + TS - 1) / TS;
   hipLaunchKernelGGL((k_tile<T>), dim3((unsigned)(ntx * nty)), dim3(NT), 0, s, (const T*)in, (const uint8_t*)dir,
                      (unsigned*)cnt, (T*)out, queue, g, ntx);
   check(hipGetLastError(), "spread tile launch");
-  const unsigned grid = (unsigned)((n + NT - 1) / NT);
-  hipLaunchKernelGGL(k_count, dim3(grid), dim3(NT), 0, s, (const uint8_t*)dir, (unsigned*)cnt, queue, g, n);
-  check(hipGetLastError(), "spread count launch");
   int dev = 0, cus = 0, per_cu = 0;
   check(hipGetDevice(&dev), "hipGetDevice");
   check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");

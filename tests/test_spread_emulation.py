@@ -1,7 +1,8 @@
-"""Host emulation of the three spreading passes of map_amd/csrc/spread.hip (k_tile,
-k_count, k_walk), run cell by cell in Python on small rasters with small tiles, against
-the oracle: checks the closed/open split, the counts and the walk hand-off (every cell
-computed once, the fixed summation order) without a GPU. Walks are run one after
+"""Host emulation of the spreading passes of map_amd/csrc/spread.hip (k_tile, k_walk), run
+cell by cell in Python on small rasters with small tiles, against the oracle: checks the
+closed/open split, the counts assembled from both sides of tile edges (contributions in a
+shuffled order, a source listed by the contribution that leaves zero) and the walk hand-off
+(every cell computed once, the fixed summation order) without a GPU. Walks are run one after
 another in a shuffled order, the schedule the atomics allow."""
 import numpy as np
 import pytest
@@ -26,6 +27,7 @@ def emulate(v, d, ts, seed):
         o = OUT.get(int(d[y, x]))
         return None if o is None or not inr(x + o[0], y + o[1]) else (x + o[0], y + o[1])
 
+    contrib = []
     # k_tile
     for ty0 in range(0, h, ts):
         for tx0 in range(0, w, ts):
@@ -72,20 +74,26 @@ def emulate(v, d, ts, seed):
             for c in cells:
                 closed = done[c] and not opn[c]
                 out[c[1], c[0]] = tot[c] if closed and down(*c) is not None else T(0)
-                cnt[c[1], c[0]] = CLOSED if closed else 0
-    # k_count
-    new = cnt.copy()
-    for y in range(h):
-        for x in range(w):
-            if cnt[y, x] == CLOSED:
-                continue
-            c = sum(1 for i1 in (-1, 0, 1) for i0 in (-1, 0, 1)
-                    if (i0 or i1) and inr(x + i0, y + i1) and d[y + i1, x + i0] == INFLOW[i1 + 1][i0 + 1]
-                    and cnt[y + i1, x + i0] != CLOSED)
-            new[y, x] = SRC if c == 0 else c
-    cnt = new
+                if closed:
+                    dn = down(*c)
+                    if dn is not None and not inside(*dn):
+                        contrib.append((dn, -1))
+                else:
+                    k = 0
+                    for i1 in (-1, 0, 1):
+                        for i0 in (-1, 0, 1):
+                            u = (c[0] + i0, c[1] + i1)
+                            if (i0 or i1) and inr(*u) and d[u[1], u[0]] == INFLOW[i1 + 1][i0 + 1]:
+                                k += 1 if (not inside(*u) or not done[u] or opn[u]) else 0
+                    contrib.append((c, k))
+    # the atomics of all tiles in any order: the one that leaves zero lists a source
+    np.random.default_rng(seed + 1).shuffle(contrib)
+    srcs = []
+    for (x, y), k in contrib:
+        cnt[y, x] += k
+        if cnt[y, x] == 0:
+            srcs.append((x, y))
     # k_walk, sources in a shuffled order
-    srcs = [(x, y) for y in range(h) for x in range(w) if cnt[y, x] == SRC]
     np.random.default_rng(seed).shuffle(srcs)
     computed = set()
     for (x, y) in srcs:
