@@ -733,8 +733,8 @@ Node* Runtime::spread_node(const Group& g) {
 
 // A spreading group is its SpreadScan node alone: the accumulation of a
 // resident raster along the resident direction raster (spread.hip: tiles in
-// LDS, a counting pass, walks from the listed sources), with a status word per
-// cell and a source list taken from the store for the three launches.
+// LDS that also count and list the walk sources, then the walks), with a count
+// word per cell and the source list taken from the store for the launches.
 void Runtime::spread_task(const Group& g, Eval& e) {
   Node* n = spread_node(g);
   if (g.nodes.size() != 1) fail("spread scan fused with other nodes");
