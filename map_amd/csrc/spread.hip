@@ -300,19 +300,37 @@ __device__ void walk(const T* __restrict__ in, const uint8_t* __restrict__ dir, 
   cell_of(g, e, x, y);
   uint64_t from = ~0ull;  // the cell the walk came from and its total
   T carried = T(0);
-  for (;;) {
-    // one round of loads for the cell and its eight neighbours (clamped into the raster;
-    // a neighbour outside it never matches), then one round for the upstream totals
-    uint64_t en[9];
-    bool inr[9];
-    hood(g, x, y, en, inr);
-    unsigned dn[9];
+  // the cell's 3x3 directions and input value (clamped into the raster; a neighbour
+  // outside it never matches); each step loads the next cell's while its own upstream
+  // totals and the next count are in flight, one round trip per step
+  uint64_t en[9];
+  bool inr[9];
+  hood(g, x, y, en, inr);
+  unsigned dn[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) dn[k] = dir[en[k]];
-    const T own = in[e];
+  for (int k = 0; k < 9; k++) dn[k] = dir[en[k]];
+  T own = in[e];
+  for (;;) {
     bool up[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) up[k] = k != 4 && inr[k] && dn[k] == inflow_code(k % 3 - 1, k / 3 - 1);
+    int dx = 0, dy = 0;
+    const bool code = outflow(dn[4], dx, dy);
+    const bool down = code && x + dx >= 0 && y + dy >= 0 && x + dx < g.DS0 && y + dy < g.DS1;
+    uint64_t ed = en[4];
+#pragma unroll
+    for (int k = 0; k < 9; k++)  // a select per slot: a computed index would put en[] in scratch memory
+      if (k == (dy + 1) * 3 + dx + 1) ed = en[k];
+    // next cell's loads (the cell itself when there is none), then this cell's totals
+    const int x2 = down ? x + dx : x, y2 = down ? y + dy : y;
+    uint64_t en2[9];
+    bool inr2[9];
+    hood(g, x2, y2, en2, inr2);
+    unsigned dn2[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) dn2[k] = dir[en2[k]];
+    const T own2 = in[ed];
+    const unsigned left = __hip_atomic_load(&cnt[ed], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     T tv[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) tv[k] = ld_total(&out[up[k] && en[k] != from ? en[k] : e]);
@@ -320,17 +338,9 @@ __device__ void walk(const T* __restrict__ in, const uint8_t* __restrict__ dir, 
 #pragma unroll
     for (int k = 0; k < 9; k++)
       if (up[k]) acc = (T)(acc + (en[k] == from ? carried : tv[k]));
-    int dx, dy;
-    if (!outflow(dn[4], dx, dy)) return;  // no downstream code: the neutral value stays
-    x += dx;
-    y += dy;
-    if (x < 0 || y < 0 || x >= g.DS0 || y >= g.DS1) return;  // flows off the raster
+    if (!down) return;  // no downstream cell inside the raster: the neutral value stays
     st_total(&out[e], acc);
-    uint64_t ed = en[4];
-#pragma unroll
-    for (int k = 0; k < 9; k++)  // a select per slot: a computed index would put en[] in scratch memory
-      if (k == (dy + 1) * 3 + dx + 1) ed = en[k];
-    if (__hip_atomic_load(&cnt[ed], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 1u) {
+    if (left != 1u) {
       // the total has reached the coherence point before the count falls, so the walk
       // that takes the count to zero reads it (its loads are issued after the atomic returns)
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -341,6 +351,15 @@ __device__ void walk(const T* __restrict__ in, const uint8_t* __restrict__ dir, 
     from = e;
     carried = acc;
     e = ed;
+    x = x2;
+    y = y2;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      en[k] = en2[k];
+      inr[k] = inr2[k];
+      dn[k] = dn2[k];
+    }
+    own = own2;
   }
 }
 
