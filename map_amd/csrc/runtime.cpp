@@ -66,17 +66,6 @@ void Store::release(void* p) {
   cached_ += it->second;
   in_use_ -= it->second;
   size_.erase(it);
-  if (cached_ > kCacheCap) {
-    // many distinct sizes (rasters of many geometries) would otherwise pile up in the
-    // cache: give the largest cached buffers back until it is under the cap
-    dev::sync();
-    while (cached_ > kCacheCap / 2 && !free_.empty()) {
-      auto last = std::prev(free_.end());
-      dev::free_device(last->second);
-      cached_ -= last->first;
-      free_.erase(last);
-    }
-  }
 }
 
 void Store::trim() {

@@ -56,7 +56,6 @@ class Store {
   size_t cached() const { return cached_; }
   ~Store();
  private:
-  static constexpr size_t kCacheCap = size_t(32) << 30;  // bytes of freed buffers kept for reuse
   std::multimap<size_t, void*> free_;
   std::unordered_map<void*, size_t> size_;
   size_t in_use_ = 0, cached_ = 0;
