@@ -96,7 +96,10 @@ GEOMS = [((512, 384), (128, 128), 1, 1, (256, 16, 4)),
          ((33, 17), (16, 16), 1, 1, (256, 16, 4)),
          ((40, 40), (40, 40), 3, 3, (256, 16, 4)),
          ((101, 57), (50, 30), 1, 1, (256, 16, 4)),
-         ((98, 60), (49, 20), 2, 1, (256, 16, 4))]
+         ((98, 60), (49, 20), 2, 1, (256, 16, 4)),
+         ((2048, 1040), (1024, 1024), 1, 1, (1024, 16, 16)),
+         ((300, 220), (128, 64), 1, 1, (512, 16, 16)),
+         ((200, 150), (64, 32), 1, 1, (256, 16, 16))]
 
 
 @pytest.mark.parametrize("ds,bs,h0,h1,tile", GEOMS)
