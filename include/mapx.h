@@ -128,7 +128,7 @@ ma_node_t *ma_spreadNeighbor(ma_node_t *node, ma_node_t *dir, ma_reduction_t typ
 /* front/bindings.hpp:77 ma_spreadScan (front/bindings.cpp:225-228): accumulation of `node` along
  * the U8 D8 directions `dir` (SUM only): each cell gets the total that leaves it towards its
  * downstream cell (its value plus all its upstream totals); cells with no downstream cell in
- * the raster, on a direction cycle or downstream of one get 0. */
+ * the raster or on a direction cycle get 0. */
 ma_node_t *ma_spreadScan(ma_node_t *node, ma_node_t *dir, ma_reduction_t type);
 /* front/bindings.hpp:79 ma_stats (front/bindings.cpp:232-235): the input's values, with its
  * maximum and minimum computed in the same evaluation (read them with ma_statsRange). */

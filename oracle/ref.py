@@ -614,8 +614,8 @@ def spread_scan(x, d, red=SUM):
     T(c) = in(c) + the sum of T(u) over the upstream neighbours u of c (the neighbours whose
     code points at c, y outer, x inner; in the raster's C type), a cell with a downstream cell
     inside the raster gets T(c) (what leaves it), any other cell 0; cells on a direction cycle
-    or downstream of one never have all their upstream totals and get 0 (the reference's
-    iteration does not end for them). Evaluated in topological order (Kahn's algorithm), one
+    never have all their upstream totals and get 0 (the reference's iteration does not end
+    for them). Evaluated in topological order (Kahn's algorithm), one
     front of ready cells at a time."""
     if red != SUM:
         raise NotImplementedError("spread_scan: SUM only")

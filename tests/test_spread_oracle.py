@@ -24,7 +24,7 @@ def _brute_scan(v, d, mod=None):
             o = off.get(int(d[y, x]))
             if o is not None and 0 <= x + o[0] < w and 0 <= y + o[1] < h:
                 nxt[(x, y)] = (x + o[0], y + o[1])
-    # cells on a cycle, and every cell downstream of one, never finish
+    # cells on a cycle never finish (a cycle cell's downstream cell is on the cycle too)
     on_cycle = set()
     for start in nxt:
         seen, c = [], start
