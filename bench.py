@@ -530,6 +530,12 @@ def main():
     M.set_option("profile", 0)
     M.set_option("graph", args.graph)
     kname, krec = max(kt.items(), key=lambda kv: kv[1]["total_ms"])
+    # every generation is one launch of one kernel: anything else means a
+    # generation ran another kernel (or none) and the roofline would not be
+    # the step's
+    if len(kt) != 1 or krec["count"] != args.gens:
+        raise SystemExit("bench.py: the profiled step launched %s, expected %d launches of one kernel"
+                         % ({k: v["count"] for k, v in kt.items()}, args.gens))
     # HBM bytes per launch of this kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE
     # passes scaled by a copy kernel of known size (profiles/life_hbm_traffic.json)
     traffic = None

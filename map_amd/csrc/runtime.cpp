@@ -541,11 +541,23 @@ std::vector<Node*> Runtime::d2_outputs(const Group& g, const std::vector<Node*>&
 // which of its nodes are kept: every generation of an iterated script shares
 // the key with its last one; nor floating mask weights, which kernels take as
 // arguments) and the block size.
+// The inputs are listed in the generated kernel's order (D2 before D0, each by
+// id), not in the fuser's: the fuser's order depends on node ids, which differ
+// between the first generation of an iterated script (its input was created
+// before the constants) and the later ones.
 std::string Runtime::tune_key(const Group& g) const {
   const Meta m = g.meta();
-  std::string sig = g.signature(false);
-  sig = sig.substr(0, sig.rfind('|'));
-  return sig + "#" + std::to_string(m.bs.arr[0]) + "x" + std::to_string(m.bs.arr[1]);
+  std::vector<Node*> ins(g.ins.begin(), g.ins.end());
+  std::sort(ins.begin(), ins.end(), [](Node* a, Node* b) {
+    if ((a->meta.nd == D0) != (b->meta.nd == D0)) return a->meta.nd != D0;
+    return a->id < b->id;
+  });
+  std::ostringstream s;
+  for (Node* n : ins) s << n->meta.nd << name_dt(n->meta.dt) << ";";
+  s << "|";
+  for (Node* n : g.nodes) s << n->signature(false) << ";";
+  s << "#" << m.bs.arr[0] << "x" << m.bs.arr[1];
+  return s.str();
 }
 
 TileShape Runtime::tile_for(const Group& g) {
