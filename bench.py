@@ -24,17 +24,6 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def life_input(side, seed):
-    import numpy as np
-    rng = np.random.Generator(np.random.PCG64(seed))
-    out = np.empty((side, side), np.uint8)
-    rows = max(1, (1 << 24) // side)
-    for r in range(0, side, rows):
-        n = min(rows, side - r)
-        out[r:r + n] = rng.random((n, side)) < 0.3
-    return out
-
-
 def life_script(M, state, gens):
     mask = [[1, 1, 1], [1, 0, 1], [1, 1, 1]]
     for _ in range(gens):
@@ -471,7 +460,8 @@ def main():
         print(json.dumps({args.only: fn(M)}), flush=True)
         return
     bs = (args.block, args.block)
-    s0 = life_input(args.side, 1)
+    from map_amd import workloads as W
+    s0 = W.life_input(args.side, 1)
     S = M.from_array(s0, bs)
     M.set_option("form", 1 if args.form == "batched" else 0)
     M.set_option("graph", args.graph)

@@ -143,6 +143,27 @@ ma_node_t *ma_access(ma_node_t *node, ma_dim2_t coord);
  * replaced by the D0 value `rhs` (of lhs's type). */
 ma_node_t *ma_lhsAccess(ma_node_t *lhs, ma_node_t *rhs, ma_dim2_t coord);
 
+/* front/bindings.hpp:56 ma_rand (front/bindings.cpp:113-116, runtime/dag/Rand.cpp): a raster of
+ * the seed raster's shape (type dt, or the seed's when dt = NONE_DATATYPE) whose cells are
+ * Philox2x32-10 outputs (Random123) with counter {seed cell, 0} and key {cell index}. */
+ma_node_t *ma_rand(ma_node_t *seed, ma_datatype_t dt, ma_memorder_t mo);
+/* front/bindings.hpp:69 ma_boundedNbh (front/bindings.cpp:172-175): declared so that callers
+ * bind; the reference generates no kernel statement for it, and here the call fails (NULL,
+ * ma_lastError). */
+ma_node_t *ma_boundedNbh(ma_node_t *node, ma_node_t *cx, ma_node_t *cy);
+/* front/bindings.hpp:86-92 symbolic-loop digestion (front/bindings.cpp:244-275,
+ * Runtime.cpp:156-311), used by python/map.py's Python 2 __nonzero__. Declared so that
+ * callers bind; here a `while` over a D0 raster runs as ordinary Python 3 (the D0
+ * condition is evaluated on the device each iteration), and these calls fail: the void
+ * ones leave ma_lastError set, ma_loopAssemble returns NULL, ma_loopAgainTail sets *num to 0. */
+void ma_loopStart(void);
+void ma_loopCond(ma_node_t *cond);
+void ma_loopBody(void);
+void ma_loopAgain(void);
+ma_node_t *ma_loopAssemble(void);
+void ma_loopEnd(void);
+void ma_loopAgainTail(ma_node_t *loop, ma_node_t ***agains, ma_node_t ***tails, int *num);
+
 /* ---- extensions (no reference counterpart) ---------------------------------- */
 
 /* Description of the last failure of a ma_* call on this thread ("" if none). */

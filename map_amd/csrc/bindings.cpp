@@ -15,10 +15,12 @@ namespace mapx { struct Node; }
 
 namespace mapx {
 namespace gen {
+size_t dem_scratch_bytes(int ng);
+size_t voronoi_scratch_bytes(int np);
 void dem(void* out, int BS0, int BS1, int NB0, int NB1, int DS0, int DS1, const double* params, int ng,
-         uint64_t seed, void* stream);
+         uint64_t seed, void* scratch, void* stream);
 void voronoi(void* out, int dt_bytes, bool is_signed, int BS0, int BS1, int NB0, int NB1, int DS0, int DS1,
-             const int* px, const int* py, int np, void* stream);
+             const int* px, const int* py, int np, void* scratch, void* stream);
 }  // namespace gen
 }  // namespace mapx
 
@@ -120,6 +122,9 @@ ma_node_t* ma_constant(ma_variant_t var, ma_dim2_t ds, ma_datatype_t dt, ma_memo
 ma_node_t* ma_cast(ma_node_t* node, ma_datatype_t type) {
   return guard([&] { return R().dag.cast(need(node), type); }, (Node*)nullptr);
 }
+ma_node_t* ma_rand(ma_node_t* seed, ma_datatype_t dt, ma_memorder_t mo) {
+  return guard([&] { return R().dag.rand(need(seed), dt, mo); }, (Node*)nullptr);
+}
 ma_node_t* ma_index(ma_node_t* node, ma_numdim_t dim) {
   return guard([&] { return R().dag.index(need(node), dim); }, (Node*)nullptr);
 }
@@ -168,6 +173,32 @@ ma_node_t* ma_radialScan(ma_node_t* node, ma_reduction_t type, ma_dim2_t coord) 
 ma_node_t* ma_zonalReduc(ma_node_t* node, ma_reduction_t type) {
   return guard([&] { return R().dag.zonalReduc(need(node), type); }, (Node*)nullptr);
 }
+ma_node_t* ma_boundedNbh(ma_node_t* node, ma_node_t* cx, ma_node_t* cy) {
+  return guard([&]() -> Node* {
+    need(node); need(cx); need(cy);
+    throw std::runtime_error("boundedNbh: not supported (the reference generates no kernel statement for it)");
+  }, (Node*)nullptr);
+}
+
+// Symbolic-loop digestion (front/bindings.cpp:244-275): a `while` over a D0
+// raster runs as ordinary Python here (Raster.__bool__), so none of these
+// builds a Loop node.
+static const char* kNoSymLoop =
+    "symbolic loops: not supported; a while over a D0 raster runs as ordinary Python (its condition is evaluated "
+    "on the device each iteration)";
+void ma_loopStart(void) { guard([]() -> int { throw std::runtime_error(kNoSymLoop); }, 0); }
+void ma_loopCond(ma_node_t*) { guard([]() -> int { throw std::runtime_error(kNoSymLoop); }, 0); }
+void ma_loopBody(void) { guard([]() -> int { throw std::runtime_error(kNoSymLoop); }, 0); }
+void ma_loopAgain(void) { guard([]() -> int { throw std::runtime_error(kNoSymLoop); }, 0); }
+ma_node_t* ma_loopAssemble(void) { return guard([]() -> Node* { throw std::runtime_error(kNoSymLoop); }, (Node*)nullptr); }
+void ma_loopEnd(void) { guard([]() -> int { throw std::runtime_error(kNoSymLoop); }, 0); }
+void ma_loopAgainTail(ma_node_t*, ma_node_t*** agains, ma_node_t*** tails, int* num) {
+  if (agains) *agains = nullptr;
+  if (tails) *tails = nullptr;
+  if (num) *num = 0;
+  guard([]() -> int { throw std::runtime_error(kNoSymLoop); }, 0);
+}
+
 ma_node_t* ma_spreadNeighbor(ma_node_t* node, ma_node_t* dir, ma_reduction_t type) {
   return guard([&] { return R().dag.spreadNeighbor(need(node), need(dir), type); }, (Node*)nullptr);
 }
@@ -499,14 +530,19 @@ ma_node_t* ma_genDem(ma_dim2_t ds, ma_dim2_t bs, const double* gauss, int ng, un
     h->nbytes = r.rasterBytes(m);
     static uint64_t uid = 1ull << 40;
     h->uid = uid++;
+    if (ng < 0 || ng > 256) throw std::runtime_error("genDem: at most 256 Gaussians");
     h->dev = r.store.alloc(h->nbytes);
+    void* scratch = nullptr;
     try {
+      scratch = r.store.alloc(gen::dem_scratch_bytes(ng));
       gen::dem(h->dev, m.bs.arr[0], m.bs.arr[1], m.nb(0), m.nb(1), m.ds.arr[0], m.ds.arr[1], gauss, ng, noise_seed,
-               dev::stream());
+               scratch, dev::stream());
     } catch (...) {
+      if (scratch) r.store.release(scratch);
       r.store.release(h->dev);
       throw;
     }
+    r.store.release(scratch);
     // the runtime owns this buffer: it is released with the node
     h->dev_owned = true;
     Node* n = r.dag.input(h);
@@ -526,14 +562,19 @@ ma_node_t* ma_genZones(ma_dim2_t ds, ma_dim2_t bs, ma_datatype_t dt, const int* 
     h->nbytes = r.rasterBytes(m);
     static uint64_t uid = 1ull << 41;
     h->uid = uid++;
+    if (np <= 0 || np > 1024) throw std::runtime_error("genZones: 1..1024 points");
     h->dev = r.store.alloc(h->nbytes);
+    void* scratch = nullptr;
     try {
+      scratch = r.store.alloc(gen::voronoi_scratch_bytes(np));
       gen::voronoi(h->dev, sizeof_dt(dt), dt == S32, m.bs.arr[0], m.bs.arr[1], m.nb(0), m.nb(1), m.ds.arr[0],
-                   m.ds.arr[1], px, py, np, dev::stream());
+                   m.ds.arr[1], px, py, np, scratch, dev::stream());
     } catch (...) {
+      if (scratch) r.store.release(scratch);
       r.store.release(h->dev);
       throw;
     }
+    r.store.release(scratch);
     h->dev_owned = true;
     Node* n = r.dag.input(h);
     r.adopt(n, h->dev);

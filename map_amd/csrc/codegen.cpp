@@ -381,6 +381,9 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
           << (n->op == D1 ? "k.bx * k.BS0 + x" : "k.by * k.BS1 + y") << ");\n"; break;
       case Kind::Cast:
         o << "    const " << T << " " << V(n) << " = (" << T << ")" << V(n->prev[0]) << ";\n"; break;
+      case Kind::Rand:  // Skeleton.cpp:183-206: counter {seed, 0}; key: the cell's linear index
+        o << "    const " << T << " " << V(n) << " = mx::rand_cell<" << T << ">((mx::u32)" << V(n->prev[0])
+          << ", (mx::u32)((mx::i64)(k.by * k.BS1 + y) * k.DS0 + (k.bx * k.BS0 + x)));\n"; break;
       case Kind::Unary: {
         auto pr = sc_pair.find(n);
         if (pr != sc_pair.end()) {

@@ -20,6 +20,7 @@ const char* kind_name(Kind k) {
     case Kind::SpreadNeighbor: return "SpreadNeighbor"; case Kind::SpreadScan: return "SpreadScan";
     case Kind::Stats: return "Stats"; case Kind::Barrier: return "Barrier";
     case Kind::Access: return "Access"; case Kind::LhsAccess: return "LhsAccess";
+    case Kind::Rand: return "Rand";
   }
   return "?";
 }
@@ -93,6 +94,7 @@ int Node::pattern() const {
   switch (kind) {
     case Kind::Cast: case Kind::Unary: case Kind::Binary: case Kind::Conditional: case Kind::Diversity:
     case Kind::Access: case Kind::LhsAccess: return LOCAL;  // Access.hpp:44, LhsAccess.hpp:45
+    case Kind::Rand: return LOCAL;  // Rand.hpp:41
     case Kind::Neighbor: case Kind::Convolution: case Kind::FocalFunc: case Kind::FocalPercent:
     case Kind::SpreadNeighbor: return FOCAL;  // SpreadNeighbor.hpp:42
     case Kind::ZonalReduc: case Kind::ZonalStats: return ZONAL;
@@ -205,6 +207,25 @@ Node* Dag::index(Node* p, int dim) {
   n->meta = p->meta; n->meta.dt = S32;
   n->op = dim;
   n->prev = {p};
+  return add(std::move(n));
+}
+
+// Rand (runtime/dag/Rand.cpp:78-86, Skeleton.cpp:183-206): the seed raster's
+// shape, type dt (the seed's type when NONE_DATATYPE); each cell is a
+// Philox2x32-10 output of counter {seed cell, 0} and key {the cell's linear
+// index y * width + x}, integers taking its low bytes and floats the first
+// word divided by UINT_MAX (F64: both words over ULONG_MAX).
+Node* Dag::rand(Node* seed, int dt, int mo) {
+  if (!seed || seed->meta.nd != D2) fail("rand: needs a D2 seed raster");
+  if (is_float_dt(seed->meta.dt) || seed->meta.dt == B8) fail("rand: the seed raster must have an integer type");
+  if (dt == NONE_DATATYPE) dt = seed->meta.dt;
+  if (dt <= NONE_DATATYPE || dt == B8 || dt >= N_DATATYPE) fail("rand: bad type");
+  auto n = std::make_unique<Node>();
+  n->kind = Kind::Rand;
+  n->meta = seed->meta;
+  n->meta.dt = dt;
+  if (mo != NONE_MEMORDER) n->meta.mo = mo;
+  n->prev = {seed};
   return add(std::move(n));
 }
 
@@ -414,6 +435,11 @@ Node* Dag::spreadScan(Node* p, Node* dir, int red) {
   if (dir->meta.dt != U8) fail("spreadScan: directions must be U8 (SpreadScan.cpp:37)");
   if (red != SUM) fail("spreadScan: only SUM is supported (the reference accumulates with +=)");
   if (p->meta.dt == B8) fail("spreadScan: needs a numeric raster");
+  // the scan's walk lists and cell indices are 32-bit element indices of the
+  // block-tiled raster (edge blocks padded to full size)
+  const uint64_t cells = (uint64_t)p->meta.nb(0) * p->meta.nb(1) * p->meta.bs.arr[0] * p->meta.bs.arr[1];
+  if (cells >= (uint64_t(1) << 32))
+    fail("spreadScan: rasters of 2^32 or more cells (edge blocks padded to whole blocks) are not supported");
   auto n = std::make_unique<Node>();
   n->kind = Kind::SpreadScan;
   n->meta = p->meta;

@@ -20,7 +20,7 @@ enum class Kind : int {
   ZonalReduc, ZonalStats,
   Read, Write, Scalar,
   RadialScan, FocalPercent, Diversity,
-  SpreadNeighbor, SpreadScan, Stats, Barrier, Access, LhsAccess
+  SpreadNeighbor, SpreadScan, Stats, Barrier, Access, LhsAccess, Rand
 };
 const char* kind_name(Kind k);
 
@@ -99,6 +99,7 @@ class Dag {
   // value-numbered node (an existing equal node when one exists).
   Node* constant(Variant v, Dim2 ds, int dt, int mo, Dim2 bs);
   Node* index(Node* p, int dim);
+  Node* rand(Node* seed, int dt, int mo);
   Node* cast(Node* p, int dt);
   Node* unary(Node* p, int op);
   Node* binary(Node* l, Node* r, int op);

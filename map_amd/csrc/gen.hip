@@ -15,6 +15,10 @@
 namespace mapx {
 namespace gen {
 
+// every product and sum rounds on its own, as in the host twin of these generators
+// (compiled without contraction), so both give the same FP64 value up to exp()
+#pragma clang fp contract(off)
+
 struct Gauss {
   double cx, cy, inv2s2, amp, r2max;
 };
@@ -110,9 +114,13 @@ static void check(hipError_t e, const char* w) {
   if (e != hipSuccess) throw std::runtime_error(std::string(w) + ": " + hipGetErrorString(e));
 }
 
-// params: ng rows of (cx, cy, sigma, amplitude).
+size_t dem_scratch_bytes(int ng) { return sizeof(Gauss) * (size_t)(ng > 0 ? ng : 1); }
+size_t voronoi_scratch_bytes(int np) { return 2 * sizeof(int) * (size_t)(np > 0 ? np : 1); }
+
+// params: ng rows of (cx, cy, sigma, amplitude). scratch: a device buffer of
+// dem_scratch_bytes(ng) (the Gaussians' table), free to reuse once this returns.
 void dem(void* out, int BS0, int BS1, int NB0, int NB1, int DS0, int DS1, const double* params, int ng,
-         uint64_t seed, void* stream) {
+         uint64_t seed, void* scratch, void* stream) {
   if (ng < 0 || ng > 256) throw std::runtime_error("gen dem: at most 256 Gaussians");
   std::vector<Gauss> h(ng > 0 ? ng : 1);
   for (int k = 0; k < ng; k++) {
@@ -123,39 +131,38 @@ void dem(void* out, int BS0, int BS1, int NB0, int NB1, int DS0, int DS1, const 
     h[k].amp = params[4 * k + 3];
     h[k].r2max = (12.0 * s) * (12.0 * s);  // exp(-72) * 3000 m is far below FP32 resolution
   }
-  Gauss* d = nullptr;
-  check(hipMalloc(&d, sizeof(Gauss) * h.size()), "hipMalloc");
+  Gauss* d = (Gauss*)scratch;
   check(hipMemcpyAsync(d, h.data(), sizeof(Gauss) * h.size(), hipMemcpyHostToDevice, (hipStream_t)stream), "H2D");
   Geo G{BS0, BS1, NB0, NB1, DS0, DS1};
   const uint64_t n = (uint64_t)BS0 * BS1 * NB0 * NB1;
   const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 65536);
   hipLaunchKernelGGL(k_dem, dim3(grid), dim3(256), 0, (hipStream_t)stream, (float*)out, G, d, ng, seed, n);
-  check(hipGetLastError(), "k_dem launch");
+  const hipError_t le = hipGetLastError();
+  // the table's host copy must outlive the copy; the scratch is reused after return
   check(hipStreamSynchronize((hipStream_t)stream), "sync");
-  check(hipFree(d), "hipFree");
+  check(le, "k_dem launch");
 }
 
+// scratch: a device buffer of voronoi_scratch_bytes(np) (the points), free to reuse once this returns.
 void voronoi(void* out, int dt_bytes, bool is_signed, int BS0, int BS1, int NB0, int NB1, int DS0, int DS1,
-             const int* px, const int* py, int np, void* stream) {
+             const int* px, const int* py, int np, void* scratch, void* stream) {
   if (np <= 0 || np > 1024) throw std::runtime_error("gen zones: 1..1024 points");
-  int *dx = nullptr, *dy = nullptr;
-  check(hipMalloc(&dx, sizeof(int) * np), "hipMalloc");
-  check(hipMalloc(&dy, sizeof(int) * np), "hipMalloc");
-  check(hipMemcpyAsync(dx, px, sizeof(int) * np, hipMemcpyHostToDevice, (hipStream_t)stream), "H2D");
-  check(hipMemcpyAsync(dy, py, sizeof(int) * np, hipMemcpyHostToDevice, (hipStream_t)stream), "H2D");
+  if (!(dt_bytes == 1 || dt_bytes == 2 || dt_bytes == 4)) throw std::runtime_error("gen zones: zone type must be 1, 2 or 4 bytes");
+  int* dx = (int*)scratch;
+  int* dy = dx + np;
+  hipStream_t s = (hipStream_t)stream;
+  check(hipMemcpyAsync(dx, px, sizeof(int) * np, hipMemcpyHostToDevice, s), "H2D");
+  check(hipMemcpyAsync(dy, py, sizeof(int) * np, hipMemcpyHostToDevice, s), "H2D");
   Geo G{BS0, BS1, NB0, NB1, DS0, DS1};
   const uint64_t n = (uint64_t)BS0 * BS1 * NB0 * NB1;
   const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 65536);
-  hipStream_t s = (hipStream_t)stream;
   if (dt_bytes == 1) hipLaunchKernelGGL(k_voronoi<uint8_t>, dim3(grid), dim3(256), 0, s, (uint8_t*)out, G, dx, dy, np, n);
   else if (dt_bytes == 2) hipLaunchKernelGGL(k_voronoi<uint16_t>, dim3(grid), dim3(256), 0, s, (uint16_t*)out, G, dx, dy, np, n);
-  else if (dt_bytes == 4 && is_signed) hipLaunchKernelGGL(k_voronoi<int32_t>, dim3(grid), dim3(256), 0, s, (int32_t*)out, G, dx, dy, np, n);
-  else if (dt_bytes == 4) hipLaunchKernelGGL(k_voronoi<uint32_t>, dim3(grid), dim3(256), 0, s, (uint32_t*)out, G, dx, dy, np, n);
-  else throw std::runtime_error("gen zones: zone type must be 1, 2 or 4 bytes");
-  check(hipGetLastError(), "k_voronoi launch");
+  else if (is_signed) hipLaunchKernelGGL(k_voronoi<int32_t>, dim3(grid), dim3(256), 0, s, (int32_t*)out, G, dx, dy, np, n);
+  else hipLaunchKernelGGL(k_voronoi<uint32_t>, dim3(grid), dim3(256), 0, s, (uint32_t*)out, G, dx, dy, np, n);
+  const hipError_t le = hipGetLastError();
   check(hipStreamSynchronize(s), "sync");
-  check(hipFree(dx), "hipFree");
-  check(hipFree(dy), "hipFree");
+  check(le, "k_voronoi launch");
 }
 
 }  // namespace gen

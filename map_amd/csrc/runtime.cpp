@@ -68,11 +68,19 @@ void Store::release(void* p) {
   size_.erase(it);
 }
 
+// A cached buffer may still be read by a launch that has been recorded for
+// graph replay but not yet enqueued (its consumer released it at launch time):
+// the recorded sequence is enqueued first, so the sync below covers it, and
+// recording resumes afterwards.
 void Store::trim() {
-  if (!free_.empty()) dev::sync();
+  if (free_.empty()) return;
+  const bool rec = dev::recording();
+  if (rec) dev::record_flush();
+  dev::sync();
   for (auto& f : free_) dev::free_device(f.second);
   free_.clear();
   cached_ = 0;
+  if (rec) dev::record_begin();
 }
 
 Store::~Store() {
@@ -811,6 +819,18 @@ bool Runtime::prepare_task(const Group& g, Eval& e, Task& t) {
   return true;
 }
 
+// Device memory outside the store (task tables, the chunk queue): when the
+// device is full, the store's cached buffers are returned and the allocation
+// is tried once more.
+static void* alloc_outside_store(Store& st, size_t bytes) {
+  try {
+    return dev::malloc_device(bytes);
+  } catch (const std::exception&) {
+    st.trim();
+    return dev::malloc_device(bytes);
+  }
+}
+
 void* Runtime::device_table(const std::string& bytes) {
   auto it = tables_.find(bytes);
   if (it != tables_.end()) return it->second;
@@ -823,7 +843,7 @@ void* Runtime::device_table(const std::string& bytes) {
     tables_.clear();
     table_bytes_ = 0;
   }
-  void* d = dev::malloc_device(std::max<size_t>(bytes.size(), 16));
+  void* d = alloc_outside_store(store, std::max<size_t>(bytes.size(), 16));
   auto ins = tables_.emplace(bytes, d).first;
   dev::h2d(d, ins->first.data(), bytes.size());  // the key keeps the source bytes alive
   table_bytes_ += bytes.size();
@@ -832,7 +852,7 @@ void* Runtime::device_table(const std::string& bytes) {
 
 void* Runtime::queue() {
   if (!queue_) {
-    queue_ = dev::malloc_device(256);
+    queue_ = alloc_outside_store(store, 256);
     static const unsigned zero[64] = {0};
     PauseRecording pause;
     dev::h2d(queue_, zero, sizeof zero);

@@ -409,7 +409,8 @@ int run(void* out, const void* in, const void* dir, void* cnt, int dt, int BS0, 
     throw std::runtime_error("spread: bad raster geometry");
   const Geo g{BS0, BS1, NB0, NB1, DS0, DS1};
   const uint64_t n = (uint64_t)BS0 * BS1 * NB0 * NB1;
-  if ((n + NT - 1) / NT > 0x7fffffffull) throw std::runtime_error("spread: raster too large for one launch");
+  // cell indices, walk sources and queue entries are 32-bit element indices
+  if (n >= (1ull << 32)) throw std::runtime_error("spread: rasters of 2^32 or more cells are not supported");
   hipStream_t s = (hipStream_t)stream;
   switch (dt) {  // DataTypeEnum: F32 1, F64 2, U8 4, U16 5, U32 6, U64 7, S8 8, S16 9, S32 10, S64 11
     case 1: launch<float>(out, in, dir, cnt, g, n, s); break;

@@ -440,6 +440,16 @@ def index(raster, dim):
     return Raster(_lib.ma_index(raster, dim))
 
 
+def rand(seed, ds=(), dt=NONE_DATATYPE, mo=NONE_MEMORDER, bs=()):
+    """Random raster (python/map.py:523-531): Philox2x32-10 of counter {seed cell, 0} and key
+    {cell index}; `seed` is an integer raster, or an int made a constant raster of size ds."""
+    if not isinstance(seed, Raster):
+        dt = S32 if dt == NONE_DATATYPE else dt
+        seed = Raster(_lib.ma_constant(Variant(seed).convert(S32), Array(list(ds)), S32,
+                                       BLK + ROW if mo == NONE_MEMORDER else mo, Array(list(bs))))
+    return Raster(_lib.ma_rand(seed, dt, mo))
+
+
 def astype(raster, dt):
     return Raster(_lib.ma_cast(raster, dt))
 
@@ -862,6 +872,7 @@ _proto('ma_blocksize', [Raster], Array)
 _proto('ma_read', [ct.c_char_p], Node)
 _proto('ma_write', [Raster, ct.c_char_p], ct.c_int)
 _proto('ma_index', [Raster, ct.c_int], Node)
+_proto('ma_rand', [Raster, ct.c_int, ct.c_int], Node)
 _proto('ma_cast', [Raster, ct.c_int], Node)
 _proto('ma_conditional', [Raster, Raster, Raster], Node)
 _proto('ma_unary', [Raster, ct.c_int], Node)

@@ -109,6 +109,18 @@ def dem_gaussians(rng, w, h, n=64, ref_side=4096.0):
                      rng.uniform(0, 3000, n)], 1)
 
 
+def life_input(side, seed, p=0.3):
+    """side x side U8 cells, Bernoulli(p) from rng.random() < p (numpy PCG64, rows in order)."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = np.empty((side, side), np.uint8)
+    rows = max(1, (1 << 24) // side)
+    for r in range(0, side, rows):
+        n = min(rows, side - r)
+        out[r:r + n] = rng.random((n, side)) < p
+    return out
+
+
 def fit_side(side, block, halo):
     """A side whose last block is empty or at least `halo` cells wide (the
     border rule reads at most one neighbour block); sides that would leave a

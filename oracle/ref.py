@@ -324,6 +324,45 @@ def astype(x, dt):
     return R(_conv(x.val, dt), dt, x.bs)
 
 
+def philox2x32_10(c0, c1, key):
+    """Philox2x32-10 (Salmon, Moraes, Dror and Shaw, SC 2011; Random123, the generator the
+    reference's Rand statement calls, runtime/skeleton/Skeleton.cpp:194-197) on uint32 arrays:
+    ten rounds of (c0, c1) <- (mulhi(M, c0) ^ key ^ c1, mullo(M, c0)), M = 0xD256D193, key
+    bumped by 0x9E3779B9 between rounds. Returns (c0, c1)."""
+    m = np.uint64(0xD256D193)
+    c0 = np.asarray(c0, np.uint32).astype(np.uint64)
+    c1 = np.asarray(c1, np.uint32).astype(np.uint64)
+    k = np.asarray(key, np.uint32).astype(np.uint64)
+    mask = np.uint64(0xFFFFFFFF)
+    for _ in range(10):
+        prod = c0 * m
+        c0, c1 = ((prod >> np.uint64(32)) ^ k ^ c1) & mask, prod & mask
+        k = (k + np.uint64(0x9E3779B9)) & mask
+    return c0.astype(np.uint32), c1.astype(np.uint32)
+
+
+def rand(seed, dt=None):
+    """Rand (runtime/dag/Rand.cpp, Skeleton.cpp:183-206): per cell Philox2x32-10 of counter
+    {seed cell, 0} and key {y * width + x}; integer types take the low bytes of the output
+    (both words for 8-byte types), F32 the first word / UINT_MAX, F64 both words / ULONG_MAX."""
+    dt = seed.dt if dt is None else dt
+    h, w = seed.val.shape
+    idx = (np.arange(h, dtype=np.uint64)[:, None] * np.uint64(w) + np.arange(w, dtype=np.uint64)[None, :])
+    sv = np.asarray(seed.val).astype(np.int64).astype(np.uint64) & np.uint64(0xFFFFFFFF)
+    c0, c1 = philox2x32_10(sv.astype(np.uint32), np.zeros_like(sv, np.uint32),
+                           (idx & np.uint64(0xFFFFFFFF)).astype(np.uint32))
+    wide = c0.astype(np.uint64) | (c1.astype(np.uint64) << np.uint64(32))
+    if dt == F32:
+        v = (c0.astype(np.float32) / np.float32(4294967295.0)).astype(np.float32)
+    elif dt == F64:
+        v = wide.astype(np.float64) / np.float64(18446744073709551615.0)
+    elif NP[dt]().itemsize == 8:
+        v = wide.view(np.int64).astype(NP[dt]) if dt == S64 else wide
+    else:
+        v = c0.astype(NP[dt])
+    return R(np.asarray(v, NP[dt]), dt, seed.bs)
+
+
 def con(c, l, r):
     c, l, r = const(c), const(l), const(r)
     if l.dt != r.dt:
@@ -398,19 +437,24 @@ def neighbor(x, coord):
 
 
 def _focal(x, lst, red):
+    """FocalFunc (runtime/skeleton/FocalSkeleton.cpp:338-372): from the reduction's neutral
+    value, OP over the window of the mask's size (y outer, x inner, border rule) in the
+    raster's type; the mask's values are not used (quirk B3). Floating MAX / MIN skip NaN
+    (fmax / fmin: the reference's OpenCL max / min leave NaN undefined)."""
     ncol, nrow = len(lst[0]), len(lst)
     h0, h1 = ncol // 2, nrow // 2
     ds = x.datasize()
     idx, nt = taps(ds, x.bs, h0, h1)
     src = np.asarray(x.val).reshape(-1)
     g = src[idx.reshape(-1, nt)]
-    if red == MAX:
-        res = g.max(axis=1)
-    elif red == MIN:
-        res = g.min(axis=1)
-    else:
-        raise NotImplementedError
-    return R(res.reshape(ds[1], ds[0]).astype(src.dtype), x.dt, x.bs)
+    acc = np.full(g.shape[0], _neutral(red, x.dt), dtype=src.dtype)
+    for t in range(nt):
+        acc = np.asarray(_red_step(acc, g[:, t], red, x.dt)).astype(src.dtype)
+    return R(acc.reshape(ds[1], ds[0]), x.dt, x.bs)
+
+
+def ffunc(x, lst, red):
+    return _focal(x, lst, red)
 
 
 AGE, ILE = 1, 2

@@ -3,7 +3,7 @@ generation (compiled with hiprtc for gfx950; no GPU needed)."""
 import numpy as np
 import pytest
 
-from tests import scripts
+from map_amd import workloads as scripts
 
 
 def dem(M, w=64, h=48, bs=(32, 16)):

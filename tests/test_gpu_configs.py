@@ -1,0 +1,244 @@
+"""GPU parity at the benchmark configurations and for the ops the other GPU tests leave out.
+
+- the Game of Life headline configuration exactly (16384 x 16384 U8, Bernoulli(0.3) from
+  seed 1, 1024 x 1024 blocks, 200 generations), bit-exact against the oracle;
+- hillshade followed by per-zone statistics over 256 Voronoi zones (the fused and the
+  reference-grouped forms), 1024 x 1024 blocks: counts bit-exact, min / max equal to the
+  extremes of the GPU's own hillshade cells, which are within the per-cell FP32 bound of the
+  FP64 oracle, FP64 sums within n * eps32 * sum|x| of the FP64 oracle's;
+- the 7x7 focal mean of the served mix, bit-exact against the FP32-semantics oracle;
+- FocalFunc (fmax / fmin / sum / prod), zonal PROD / rAND / rOR and Rand against the oracle;
+- every autotuner candidate tile shape gives identical bits (Life, hillshade);
+- the HIP input generators against their CPU twins in the oracle.
+"""
+
+import numpy as np
+import pytest
+
+from map_amd import workloads as W
+from tests.parity import assert_exact, fp32_report
+
+pytestmark = pytest.mark.gpu
+
+
+def test_life_headline_config(G, O):
+    side, bs, gens = 16384, (1024, 1024), 200
+    print("case life %dx%d blocks %s, %d generations, seed 1" % (side, side, bs, gens), flush=True)
+    s0 = W.life_input(side, 1)
+    got = G.to_array(W.life(G, G.from_array(s0, bs), gens))
+    ref = O.life(s0, bs, gens)
+    assert_exact(got, ref, "life 16384^2 x 200")
+    assert 0 < int(got.sum()) < side * side
+
+
+def _c2_inputs(G, side, block, nz):
+    rng = np.random.Generator(np.random.PCG64(2))
+    gauss = W.dem_gaussians(rng, side, side, ref_side=4096.0 * side / 4096)
+    dem = G.gen_dem((side, side), (block, block), gauss, 2)
+    rz = np.random.Generator(np.random.PCG64(3))
+    px, py = rz.integers(0, side, nz), rz.integers(0, side, nz)
+    zones = G.gen_zones((side, side), (block, block), px, py)
+    return dem, zones
+
+
+@pytest.mark.parametrize("fused", [1, 0])
+def test_hillshade_zonal_stats_config(G, O, fused):
+    """Hillshade -> per-zone statistics over 256 Voronoi zones, 4096 x 4096 F32 DEM of the
+    benchmark's generator, 1024 x 1024 blocks, fused (focal_zonal) and in the reference's
+    grouping (hillshade materialised)."""
+    side, block, nz = 4096, 1024, 256
+    print("case hillshade+zstats %dx%d blocks %d, %d zones, fused=%d" % (side, side, block, nz, fused), flush=True)
+    dem, zones = _c2_inputs(G, side, block, nz)
+    try:
+        G.set_option("focal_zonal", fused)
+        zs = W.hillshade_zonal(G, dem, zones, nz)
+        G.eval(zs)
+        plan = G.plan(W.hillshade_zonal(G, dem, zones, nz))
+    finally:
+        G.set_option("focal_zonal", 0)
+    kernels = [g for g in plan["groups"] if "kernel" in g]
+    assert len(kernels) == (1 if fused else 2), [g["kinds"] for g in kernels]
+    t = zs.table()
+    hs_gpu = G.to_array(W.hillshade(G, dem)).astype(np.float64)  # the cells the zonal group reduced
+    d = G.to_array(dem)
+    z = G.to_array(zones)
+    zf = z.reshape(-1)
+    # counts: bit-exact
+    cnt = np.bincount(zf, minlength=nz)
+    assert np.array_equal(t["count"], cnt)
+    assert int(t["count"].sum()) == side * side
+    # min / max: exactly the extremes of the GPU's hillshade cells in each zone
+    order = np.argsort(zf, kind="stable")
+    bounds = np.searchsorted(zf[order], np.arange(nz + 1))
+    hv = hs_gpu.reshape(-1)[order]
+    for k in range(nz):
+        seg = hv[bounds[k]:bounds[k + 1]]
+        assert t["min"][k] == seg.min() and t["max"][k] == seg.max(), k
+    # the GPU's cells against the FP64 oracle: within what the FP32 evaluation itself reaches
+    od = O.from_array(d, (block, block))
+    ref32 = W.hillshade(O, od).val.astype(np.float64)
+    O.set_mode("f64")
+    t1, t2 = W.hillshade_terms(O, od)
+    ref64 = np.asarray((t1 + t2).val, np.float64)
+    O.set_mode("f32")
+    rg, ro = fp32_report(hs_gpu, ref64, np.abs(t1.val) + np.abs(t2.val)), fp32_report(ref32, ref64,
+                                                                                      np.abs(t1.val) + np.abs(t2.val))
+    print("  cells gpu vs fp64: %s | fp32 oracle vs fp64: %s" % (rg, ro), flush=True)
+    cell_bound = 2 * ro["max_abs"] + 1e-6
+    assert rg["max_abs"] <= cell_bound
+    # sums: FP64 accumulation of the FP32 cells, against the oracle's exact sums of the same cells
+    own = O.zstats(O.from_array(hs_gpu.astype(np.float32), (block, block)), O.from_array(z, (block, block)), nz)
+    eps32 = float(np.finfo(np.float32).eps)
+    absum = np.bincount(zf, weights=np.abs(hs_gpu.reshape(-1)), minlength=nz)
+    assert np.all(np.abs(t["sum"] - own["sum"]) <= 64 * np.finfo(np.float64).eps * absum + 1e-12)
+    # and against the FP64 oracle's hillshade: n * eps32 * sum|x| plus n times the cell bound
+    r64 = O.zstats(O.from_array(ref64, (block, block)), O.from_array(z, (block, block)), nz)
+    bound = cnt * eps32 * absum + cnt * cell_bound
+    err = np.abs(t["sum"] - r64["sum"])
+    print("  sums: max |gpu - fp64 oracle| %.3g (bound min %.3g)" % (err.max(), bound.min()), flush=True)
+    assert np.all(err <= bound)
+    assert np.allclose(t["mean"], t["sum"] / np.maximum(cnt, 1), rtol=0, atol=1e-12)
+
+
+def test_focal_mean7_matches_oracle(G, O):
+    """The 7x7 focal mean of the served mix (convolve with a 7x7 S32 mask of ones, / 49) on a
+    ragged DEM in 256 x 128 blocks: bit-exact against the FP32-semantics oracle."""
+    print("case focal mean 7x7 777x515 blocks 256x128", flush=True)
+    w, h, bs = 777, 515, (256, 128)
+    rng = np.random.Generator(np.random.PCG64(17))
+    g = np.stack([rng.uniform(0, w, 12), rng.uniform(0, h, 12), rng.uniform(8, 80, 12), rng.uniform(0, 3000, 12)], 1)
+    dem = O.dem(0, 0, w, h, g, 17)
+    got = G.to_array(W.focal_mean7(G, G.from_array(dem, bs)))
+    ref = W.focal_mean7(O, O.from_array(dem, bs)).val
+    assert got.dtype == np.float32
+    assert_exact(got, ref, "7x7 mean")
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.int32, np.uint8, np.float64])
+def test_focal_func_matches_oracle(G, O, dt):
+    """ma_focalFunc MAX / MIN (and SUM / PROD) over 3x3, 5x3 and 3x5 windows on a ragged raster
+    (block seams, raster edges, the border rule), the mask's values ignored; NaN among the
+    floats (skipped by MAX / MIN); bit-exact against the oracle."""
+    print("case focal func", np.dtype(dt).name, flush=True)
+    w, h, bs = 130, 90, (64, 32)
+    rng = np.random.default_rng(31)
+    if np.issubdtype(dt, np.floating):
+        a = rng.normal(0, 10, (h, w)).astype(dt)
+        a[rng.random((h, w)) < 0.03] = np.nan
+    else:
+        a = rng.integers(0, 200, (h, w)).astype(dt)
+    A, oa = G.from_array(a, bs), O.from_array(a, bs)
+    masks = ([[1, 2, 3]] * 3, [[0.5] * 5] * 3, [[-1, 0, 1]] * 5)
+    for mask in masks:
+        for red in (G.MAX, G.MIN):
+            got = G.to_array(G._focal(A, mask, red))
+            assert_exact(got, O.ffunc(oa, mask, red).val, "ffunc %d %dx%d" % (red, len(mask[0]), len(mask)))
+    small = (a % 3 + 1).astype(dt) if not np.issubdtype(dt, np.floating) else np.round(a / 16, 2).astype(dt)
+    S, os_ = G.from_array(small, bs), O.from_array(small, bs)
+    for red in (G.SUM, G.PROD):
+        assert_exact(G.to_array(G._focal(S, [[1] * 3] * 3, red)), O.ffunc(os_, [[1] * 3] * 3, red).val,
+                     "ffunc %d" % red)
+
+
+@pytest.mark.parametrize("dt", [np.uint8, np.int32, np.uint16, np.float32, np.float64])
+def test_zonal_prod_and_or_match_oracle(G, O, dt):
+    """zonal PROD (integer types wrap in the type), rAND and rOR over ragged rasters in
+    several blocks, against the oracle; floating products accumulate in FP64 in a fixed order,
+    within n * eps of the oracle's FP64 product."""
+    print("case zonal prod/and/or", np.dtype(dt).name, flush=True)
+    w, h, bs = 203, 150, (64, 64)
+    rng = np.random.default_rng(32)
+    if np.issubdtype(dt, np.floating):
+        a = rng.uniform(0.97, 1.03, (h, w)).astype(dt)
+    else:
+        a = rng.integers(1, 200, (h, w)).astype(dt) | 1
+    A, oa = G.from_array(a, bs), O.from_array(a, bs)
+    reds = [G.PROD] + ([] if np.issubdtype(dt, np.floating) else [G.rAND, G.rOR])
+    rs = [G.Raster(G._lib.ma_zonalReduc(A, red)) for red in reds]
+    G.eval(*rs)
+    for red, r in zip(reds, rs):
+        got = G.value(r)
+        want = O.value(O._zred(oa, red))
+        if np.issubdtype(dt, np.floating):
+            n = a.size
+            assert abs(got - want) <= n * np.finfo(dt).eps * abs(want) + 1e-300, (red, got, want)
+        else:
+            assert got == want, (red, got, want)
+    # masks whose rAND / rOR are not trivial
+    if not np.issubdtype(dt, np.floating):
+        b = (rng.integers(0, 2, (h, w)) * 0x5 | 0x2).astype(dt)
+        B, ob = G.from_array(b, bs), O.from_array(b, bs)
+        ra, ro = G.Raster(G._lib.ma_zonalReduc(B, G.rAND)), G.Raster(G._lib.ma_zonalReduc(B, G.rOR))
+        G.eval(ra, ro)
+        assert G.value(ra) == O.value(O._zred(ob, O.rAND)) == 0x2
+        assert G.value(ro) == O.value(O._zred(ob, O.rOR)) == 0x7
+
+
+@pytest.mark.parametrize("dt", ["U8", "S16", "U32", "S32", "U64", "F32", "F64"])
+def test_rand_matches_oracle(G, O, dt):
+    """rand(seed) (Philox2x32-10 per cell) on a ragged raster in several blocks, seeds from an
+    integer raster, fused with a consumer: bit-exact against the oracle."""
+    print("case rand", dt, flush=True)
+    w, h, bs = 300, 170, (128, 64)
+    seed = np.random.default_rng(33).integers(-2 ** 31, 2 ** 31, (h, w)).astype(np.int32)
+    S, os_ = G.from_array(seed, bs), O.from_array(seed, bs)
+    code = getattr(G, dt)
+    r = G.rand(S, dt=code)
+    got = G.to_array(r)
+    want = O.rand(os_, getattr(O, dt)).val
+    assert got.dtype == want.dtype
+    assert_exact(got, want, "rand " + dt)
+    both = G.to_array(G.rand(S, dt=code) + G.rand(S * 1, dt=code))
+    assert_exact(both, (O.rand(os_, getattr(O, dt)) + O.rand(os_ * 1, getattr(O, dt))).val, "rand fused " + dt)
+
+
+def test_every_autotuner_candidate_gives_identical_bits(G, O):
+    """Life and hillshade evaluated with each candidate tile shape of the autotuner (LDS-staged,
+    prefetching and register-window forms): identical results; Life equal to the oracle."""
+    rng = np.random.Generator(np.random.PCG64(34))
+    s0 = (rng.random((520, 700)) < 0.3).astype(np.uint8)
+    bs = (256, 128)
+    from tests.test_gpu_parity import dem_array
+    dem = dem_array(700, 520, 35)
+    rep = G.autotune(W.life(G, G.from_array(s0, bs), 2), reps=1)
+    cands = sorted({tuple(c["tile"]) for c in rep["candidates"]})
+    print("case autotuner candidates", len(cands), flush=True)
+    assert len(cands) >= 20
+    life_ref = O.life(s0, bs, 6)
+    hs0 = None
+    try:
+        for c in cands:
+            for key, val in zip(("tile_tx", "tile_ty", "tile_vx", "tile_pf", "tile_st"), c):
+                G.set_option(key, val)
+            got = G.to_array(W.life(G, G.from_array(s0, bs), 6))
+            assert_exact(got, life_ref, "life tile %s" % (c,))
+            hs = G.to_array(W.hillshade(G, G.from_array(dem, bs)))
+            if hs0 is None:
+                hs0 = hs
+            assert_exact(hs, hs0, "hillshade tile %s" % (c,))
+    finally:
+        for key in ("tile_tx", "tile_ty", "tile_vx", "tile_pf", "tile_st"):
+            G.set_option(key, 0)
+
+
+def test_generators_match_their_cpu_twins(G, O):
+    """The HIP generators (gen_dem, gen_zones) against the oracle's restatement on a ragged
+    raster in blocks: zones bit-exact (integer arithmetic); DEM cells equal after the single
+    FP64 -> FP32 rounding except where the device's and glibc's FP64 exp() differ in the last
+    bit and the sum lies on a rounding boundary: at most 1 FP32 ulp, in a tiny share of cells."""
+    w, h, bs = 1500, 1100, (512, 256)
+    print("case generators %dx%d blocks %s" % (w, h, bs), flush=True)
+    rng = np.random.Generator(np.random.PCG64(36))
+    gauss = W.dem_gaussians(rng, w, h)
+    px, py = rng.integers(0, w, 97), rng.integers(0, h, 97)
+    gd = G.to_array(G.gen_dem((w, h), bs, gauss, 36))
+    od = O.dem(0, 0, w, h, gauss, 36)
+    sp = np.spacing(np.abs(od)).astype(np.float64)
+    ulp = np.abs(gd.astype(np.float64) - od.astype(np.float64)) / sp
+    diff = int(np.count_nonzero(gd != od))
+    print("  dem: %d of %d cells differ, max %.1f ulp" % (diff, gd.size, ulp.max()), flush=True)
+    assert ulp.max() <= 1.0 and diff <= gd.size // 10000
+    for dt in (G.S32, G.U8, G.U16):
+        gz = G.to_array(G.gen_zones((w, h), bs, px, py, dt))
+        oz = O.voronoi(0, 0, w, h, px, py)
+        assert_exact(gz.astype(np.int64), oz.astype(np.int64) % (256 if dt == G.U8 else 1 << 32), "zones %d" % dt)

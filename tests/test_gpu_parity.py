@@ -7,7 +7,7 @@ against the FP32-semantics oracle for reference.
 import numpy as np
 import pytest
 
-from tests import scripts
+from map_amd import workloads as scripts
 from tests.parity import assert_exact, assert_fp32, fp32_report
 
 pytestmark = pytest.mark.gpu

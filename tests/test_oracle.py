@@ -9,7 +9,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from tests import scripts
+from map_amd import workloads as scripts
 
 
 # ---- border rule (skeleton/util.cpp:492-506), derived by hand -------------------

@@ -7,7 +7,7 @@ another in a shuffled order, the schedule the atomics allow."""
 import numpy as np
 import pytest
 
-from tests import scripts
+from map_amd import workloads as scripts
 
 INFLOW = [[2, 4, 8], [1, 0, 16], [128, 64, 32]]
 OUT = {INFLOW[i1 + 1][i0 + 1]: (-i0, -i1) for i1 in (-1, 0, 1) for i0 in (-1, 0, 1) if i0 or i1}

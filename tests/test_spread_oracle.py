@@ -10,7 +10,7 @@ neighbour and conditional ops (front/Raster.cpp:583-606).
 import numpy as np
 import pytest
 
-from tests import scripts
+from map_amd import workloads as scripts
 
 
 def _brute_scan(v, d, mod=None):
