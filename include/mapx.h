@@ -195,7 +195,9 @@ int ma_release(ma_node_t *node);
  * 2 never), "chunk_tiles" (tiles a work-group takes at a time in groups
  * without reductions; 0: 16), "tile_pf" (focal groups load the next tile
  * while computing one), "tile_st" (focal groups whose halo inputs are rasters
- * hold their rows in registers instead of staging tiles in LDS; default 0).
+ * hold their rows in registers instead of staging tiles in LDS; default 0),
+ * "libm" (FP32 sqrt, atan2 and sin/cos pairs call the device library's
+ * functions instead of the fast paths, which are within 4 ulp; default 0).
  * Returns 0 on success. */
 int ma_setOption(const char *key, long long value);
 long long ma_getOption(const char *key);
@@ -249,6 +251,13 @@ int ma_tuneFile(const char *path, int save);
 /* Fusion legality of a pattern pair (PatternEnum bit sets): bit 0 pipe, bit 1
  * flat (runtime/Pattern.cpp:53-182 as applied by the fuser). */
 int ma_fusionRules(int top_or_left, int bot_or_right);
+/* Accuracy of the FP32 fast paths of generated kernels against FP64 on the device: fn "sqrt",
+ * "sin", "cos", "sincos" (every FP32 bit pattern in [lo, hi), hi <= 2^32), "atan2" / "atan2_near"
+ * (each pattern x paired with a y of random bits, or of x's sign and exponent). Gives the largest
+ * error in ulps (of the FP32 value nearest the FP64 result), the count of results above 4 ulp,
+ * the count checked and the input bits of the largest error; 0 on success. */
+int ma_mathCheck(const char *fn, unsigned long long lo, unsigned long long hi, double *max_ulp,
+                 unsigned long long *n_over, unsigned long long *n, unsigned *worst_in);
 
 #ifdef __cplusplus
 }

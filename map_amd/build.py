@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libmapx.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HOST_SOURCES = ["dag.cpp", "fuser.cpp", "codegen.cpp", "runtime.cpp", "serve.cpp", "rasterio.cpp", "dev.cpp", "bindings.cpp"]
-DEVICE_SOURCES = ["gen.hip", "radial.hip", "spread.hip"]
+DEVICE_SOURCES = ["gen.hip", "radial.hip", "spread.hip", "mathcheck.hip"]
 HIP_DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 CXXFLAGS = ["-O2", "-g0", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-D__HIP_PLATFORM_AMD__",
             "-I" + os.path.join(ROCM, "include"), "-I" + BUILD, "-I" + CSRC]

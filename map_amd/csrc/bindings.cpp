@@ -22,6 +22,10 @@ void dem(void* out, int BS0, int BS1, int NB0, int NB1, int DS0, int DS1, const 
 void voronoi(void* out, int dt_bytes, bool is_signed, int BS0, int BS1, int NB0, int NB1, int DS0, int DS1,
              const int* px, const int* py, int np, void* scratch, void* stream);
 }  // namespace gen
+namespace mathcheck {
+void run(const std::string& name, uint64_t lo, uint64_t hi, void* scratch, void* stream, double* max_ulp,
+         unsigned long long* n_over, unsigned long long* n, unsigned* worst_in);
+}  // namespace mathcheck
 }  // namespace mapx
 
 using namespace mapx;
@@ -334,6 +338,7 @@ int ma_setOption(const char* key, long long value) {
     else if (k == "tile_vx") o.tile.VX = (int)value;
     else if (k == "tile_pf") o.tile.PF = value != 0;
     else if (k == "tile_st") o.tile.ST = value != 0;
+    else if (k == "libm") o.libm = value != 0;
     else if (k == "wg_per_cu") {
       if (value < 0 || value > 64) throw std::runtime_error("setOption: wg_per_cu in [0, 64]");
       o.wg_per_cu = (int)value;
@@ -363,6 +368,7 @@ long long ma_getOption(const char* key) {
   if (k == "tile_vx") return o.tile.VX;
   if (k == "tile_pf") return o.tile.PF;
   if (k == "tile_st") return o.tile.ST;
+  if (k == "libm") return o.libm;
   if (k == "wg_per_cu") return o.wg_per_cu;
   if (k == "batch_tasks") return o.batch_tasks;
   if (k == "queue") return o.queue;
@@ -580,6 +586,24 @@ ma_node_t* ma_genZones(ma_dim2_t ds, ma_dim2_t bs, ma_datatype_t dt, const int* 
     r.adopt(n, h->dev);
     return n;
   }, (Node*)nullptr);
+}
+
+int ma_mathCheck(const char* fn, unsigned long long lo, unsigned long long hi, double* max_ulp,
+                 unsigned long long* n_over, unsigned long long* n, unsigned* worst_in) {
+  return guard([&] {
+    if (!dev::initialized()) throw std::runtime_error("mathCheck: no device (call ma_setupDevices)");
+    if (!fn || !max_ulp || !n_over || !n || !worst_in) throw std::runtime_error("mathCheck: null argument");
+    Runtime& r = R();
+    void* scratch = r.store.alloc(64);
+    try {
+      mathcheck::run(fn, lo, hi, scratch, dev::stream(), max_ulp, n_over, n, worst_in);
+    } catch (...) {
+      r.store.release(scratch);
+      throw;
+    }
+    r.store.release(scratch);
+    return 0;
+  }, -1);
 }
 
 }  // extern "C"

@@ -49,10 +49,13 @@ int acc_type(int red, int dt) { return (is_float_dt(dt) && (red == SUM || red ==
 
 std::string lit(const Variant& v) { return variant_literal(v); }
 
-std::string unary_stmt(const Node* n, const std::string& p, int pdt) {
+// `fast`: FP32 sqrt goes through mx::sqrt_f32 (v_sqrt_f32 on its normal
+// range, within 1 ulp) instead of the correctly rounded library sqrtf.
+std::string unary_stmt(const Node* n, const std::string& p, int pdt, bool fast) {
   const int op = n->op, dt = n->meta.dt;
   if (unary_is_operator(op)) return std::string("(") + CT(dt) + ")(" + unary_code(op) + p + ")";
   if (op == ABS && !is_float_dt(dt)) return std::string("(") + CT(dt) + ")(" + p + " < 0 ? -" + p + " : " + p + ")";
+  if (fast && dt == F32 && op == SQRT) return "mx::sqrt_f32(" + p + ")";
   std::string fn = op == ABS ? "fabs" : unary_code(op);
   if (dt == F32) return fn + "f(" + p + ")";
   if (dt == F64) return fn + "(" + p + ")";
@@ -60,7 +63,9 @@ std::string unary_stmt(const Node* n, const std::string& p, int pdt) {
   return std::string("(") + CT(dt) + ")" + fn + "((double)" + p + ")";
 }
 
-std::string binary_stmt(const Node* n, const std::string& l, const std::string& r) {
+// `fast`: FP32 atan2 goes through mx::atan2_f32 (within 4 ulp; the library
+// atan2f outside its fast domain).
+std::string binary_stmt(const Node* n, const std::string& l, const std::string& r, bool fast) {
   const int op = n->op, dt = n->meta.dt;
   const int ldt = n->prev[0]->meta.dt, rdt = n->prev[1]->meta.dt;
   if (op == MOD && (is_float_dt(ldt) || is_float_dt(rdt))) {
@@ -81,6 +86,7 @@ std::string binary_stmt(const Node* n, const std::string& l, const std::string& 
     return std::string("(") + L + (op == MAX2 ? " > " : " < ") + R + " ? " + L + " : " + R + ")";
   }
   std::string fn = binary_code(op);
+  if (fast && dt == F32 && op == ATAN2) return "mx::atan2_f32(" + L + ", " + R + ")";
   if (dt == F32) return fn + "f(" + L + ", " + R + ")";
   if (dt == F64) return fn + "(" + L + ", " + R + ")";
   return std::string("(") + CT(dt) + ")" + fn + "((double)" + l + ", (double)" + r + ")";
@@ -135,7 +141,8 @@ static std::string reduce_stmt(int op, int dt, const std::string& acc, const std
   return o.str();
 }
 
-KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, TileShape tile) {
+KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, TileShape tile, bool libm) {
+  const bool fast = !libm;
   KernelSpec ks;
   ks.tile = tile;
   ks.d2_outs = d2_outs;
@@ -391,16 +398,16 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
           const Node* cn = n->op == COS ? n : pr->second;
           if (n->id < pr->second->id) {
             o << "    " << T << " " << V(sn) << "_s, " << V(cn) << "_c;\n";
-            o << "    " << (T == std::string("float") ? "sincosf(" : "sincos(") << V(n->prev[0]) << ", &" << V(sn) << "_s, &"
+            o << "    " << (T == std::string("float") ? (fast ? "mx::sincos_f32(" : "sincosf(") : "sincos(") << V(n->prev[0]) << ", &" << V(sn) << "_s, &"
               << V(cn) << "_c);\n";
           }
           o << "    const " << T << " " << V(n) << " = " << V(n) << (n->op == SIN ? "_s" : "_c") << ";\n";
           break;
         }
-        o << "    const " << T << " " << V(n) << " = " << unary_stmt(n, V(n->prev[0]), n->prev[0]->meta.dt) << ";\n"; break;
+        o << "    const " << T << " " << V(n) << " = " << unary_stmt(n, V(n->prev[0]), n->prev[0]->meta.dt, fast) << ";\n"; break;
       }
       case Kind::Binary:
-        o << "    const " << T << " " << V(n) << " = " << binary_stmt(n, V(n->prev[0]), V(n->prev[1])) << ";\n"; break;
+        o << "    const " << T << " " << V(n) << " = " << binary_stmt(n, V(n->prev[0]), V(n->prev[1]), fast) << ";\n"; break;
       case Kind::Conditional:
         o << "    const " << T << " " << V(n) << " = (" << V(n->prev[0]) << ") ? " << V(n->prev[1]) << " : "
           << V(n->prev[2]) << ";\n"; break;
@@ -414,10 +421,19 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
         const bool marg = is_float_dt(n->mask.dtype());  // weights from P::Args (packed %f-quantised)
         for (int i1 = -h1; i1 <= h1; i1++)
           for (int i0 = -h0; i0 <= h0; i0++) {
+            const Variant wq = quantize_literal(n->mask.at(i0 + h0, i1 + h1));
             std::string w = marg ? "a.m" + LN(n) + "[" + std::to_string((i1 + h1) * n->mask.ds.arr[0] + (i0 + h0)) + "]"
-                                 : lit(quantize_literal(n->mask.at(i0 + h0, i1 + h1)));
-            o << "    " << V(n) << " += W.w" << LN(n->prev[0]) << "[H1 + (" << i1 << ")][PAD + v + (" << i0 << ")] * "
-              << w << ";\n";
+                                 : lit(wq);
+            const std::string tap = "W.w" + LN(n->prev[0]) + "[H1 + (" + std::to_string(i1) + ")][PAD + v + (" +
+                                    std::to_string(i0) + ")]";
+            if (!marg && is_float_dt(n->meta.dt) && variant_as<double>(wq) == 0.0) {
+              // a zero weight: x * 0 is exact (+-0, or NaN for inf / NaN taps), so one
+              // fused multiply-add gives the bits of the multiply and the add
+              o << "    " << V(n) << " = " << (n->meta.dt == F32 ? "__builtin_fmaf((float)" : "__builtin_fma((double)") << tap
+                << ", " << (n->meta.dt == F32 ? "0.0f" : "0.0") << ", " << V(n) << ");\n";
+            } else {
+              o << "    " << V(n) << " += " << tap << " * " << w << ";\n";
+            }
           }
         break;
       }

@@ -69,7 +69,9 @@ struct KernelSpec {
 //          const int* starts, int n_single, int stride, int F)
 // Builds the kernel of group `g`. `d2_outs` are the D2 nodes the group must
 // write to HBM (group outputs consumed later, written to files or requested).
-KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, TileShape tile);
+// `libm`: FP32 sqrt, atan2 and sin/cos pairs call the device library instead of
+// the fast paths of mapx_dev.hiph (mx::sqrt_f32, mx::atan2_f32, mx::sincos_f32).
+KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, TileShape tile, bool libm = false);
 
 // Default tile shape for a group: VX cells per thread along x so that a
 // wavefront row covers >= 256 contiguous bytes.

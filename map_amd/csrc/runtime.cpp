@@ -1089,7 +1089,7 @@ std::string Runtime::plan_key(const std::vector<Node*>& list, const std::vector<
   std::string k;
   k.reserve(list.size() * 9 + targets.size() * 8 + 16);
   auto put = [&](int v) { k.append((const char*)&v, sizeof v); };
-  put(opt.code_fusion); put(opt.focal_zonal); put(opt.tile.TX); put(opt.tile.TY); put(opt.tile.VX); put(opt.tile.PF); put(opt.tile.ST);
+  put(opt.code_fusion); put(opt.focal_zonal); put(opt.tile.TX); put(opt.tile.TY); put(opt.tile.VX); put(opt.tile.PF); put(opt.tile.ST); put(opt.libm);
   for (Node* n : list) { put(n->id); k.push_back(n->cut ? 'c' : (n->isOutput() ? 'o' : 'n')); }
   k.push_back('|');
   for (Node* t : targets) put(t->id);
@@ -1102,7 +1102,7 @@ Runtime::KernelEntry& Runtime::kernel_for(Plan* plan, const Group& g, const std:
   if (it != plan->kernels.end()) return it->second;
   HostTimer ht(cnt.ms_codegen);
   KernelEntry e;
-  e.spec = generate_kernel(g, kouts, tile_for(g));
+  e.spec = generate_kernel(g, kouts, tile_for(g), opt.libm);
   kernel_group_[e.spec.name] = tune_key(g);
   kernel_src_[e.spec.name] = e.spec.source;
   dev::Function f = dev::function(e.spec.source, e.spec.name);
@@ -1274,7 +1274,7 @@ std::string Runtime::plan(const std::vector<Node*>& targets_in) {
         KernelSpec ks;
         try {
           HostTimer ht(cnt.ms_codegen);
-          ks = generate_kernel(*g, kouts, tile_for(*g));
+          ks = generate_kernel(*g, kouts, tile_for(*g), opt.libm);
         } catch (const std::exception& e) {
           // the grouping is still reported; evaluation would fail with this error
           j << ", \"error\": \"" << esc(e.what()) << "\"}";

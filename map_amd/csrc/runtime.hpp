@@ -34,6 +34,7 @@ struct Options {
   int queue = 0;              // device chunk queue: 0 for batched launches only, 1 always, 2 never
   int chunk_tiles = 0;        // tiles per chunk of non-zonal groups (0: 16)
   TileShape tile{0, 0, 0, 0, 0}; // forced tile shape (TX 0 = per-group default / tuned)
+  bool libm = false;          // FP32 sqrt / atan2 / sin-cos pairs through the device library, not the fast paths
 };
 
 struct Counters {

@@ -841,6 +841,17 @@ class Server:
             _err(None)
 
 
+def math_check(fn, lo=0, hi=1 << 32):
+    """Accuracy of an FP32 fast path of the generated kernels (sqrt, sin, cos, sincos, atan2,
+    atan2_near) over FP32 bit patterns [lo, hi) against FP64 on the device:
+    {max_ulp, n_over_4ulp, n, worst_input_bits}."""
+    mu, no, n, wi = ct.c_double(), ct.c_ulonglong(), ct.c_ulonglong(), ct.c_uint()
+    if _lib.ma_mathCheck(fn.encode(), ct.c_ulonglong(lo), ct.c_ulonglong(hi), ct.byref(mu), ct.byref(no),
+                         ct.byref(n), ct.byref(wi)) != 0:
+        _err(None)
+    return {"max_ulp": mu.value, "n_over_4ulp": no.value, "n": n.value, "worst_input_bits": wi.value}
+
+
 def kernel_source(name):
     """Generated HIP source of a kernel built in this process."""
     s = _lib.ma_kernelSource(name.encode())
@@ -908,6 +919,8 @@ _proto('ma_spreadScan', [Raster, Raster, ct.c_int], Node)
 _proto('ma_lastEvalMs', [], ct.c_double)
 _proto('ma_kernelTimes', [], ct.c_char_p)
 _proto('ma_kernelSource', [ct.c_char_p], ct.c_char_p)
+_proto('ma_mathCheck', [ct.c_char_p, ct.c_ulonglong, ct.c_ulonglong, ct.POINTER(ct.c_double), ct.POINTER(ct.c_ulonglong),
+                        ct.POINTER(ct.c_ulonglong), ct.POINTER(ct.c_uint)], ct.c_int)
 _proto('ma_serveOpen', [ct.c_longlong, ct.c_int], ct.c_int)
 _proto('ma_serveSubmit', [ct.POINTER(Node), ct.c_int], ct.c_longlong)
 _proto('ma_serveStep', [ct.c_int], ct.c_int)

@@ -32,9 +32,10 @@ def _ulps(got, ref):
 def test_hillshade(G, O, w, h, bs):
     """Hillshade through the HIP path vs the reference-semantics FP32 oracle.
 
-    Exact FP32 stages (Sobel sums in reference order, /8, x*x+y*y, sqrt) are
-    bit-exact; every transcendental node (atan, atan2, sin, cos) is within 4
-    ULP of the correctly rounded FP32 result on the same FP32 input; the
+    Exact FP32 stages (Sobel sums in reference order, /8, x*x+y*y) are
+    bit-exact; sqrt within 1 ULP and every transcendental node (atan, atan2,
+    sin, cos) within 4 ULP of the correctly rounded FP32 result on the same
+    FP32 input; the
     composed output's error against the FP64 oracle is reported for both the
     GPU and the FP32-semantics oracle.
     """
@@ -42,11 +43,15 @@ def test_hillshade(G, O, w, h, bs):
     dem = dem_array(w, h, 7)
     D = G.from_array(dem, bs)
     od = O.from_array(dem, bs)
-    # exact stages: bit-exact
-    r_g = G.to_array(G.sqrt(scripts.hori(G, D) * scripts.hori(G, D) + scripts.vert(G, D) * scripts.vert(G, D)))
+    # exact stages: bit-exact; sqrt (v_sqrt_f32 on its normal range): within 1 ulp
+    m_g = G.to_array(scripts.hori(G, D) * scripts.hori(G, D) + scripts.vert(G, D) * scripts.vert(G, D))
     x_o, y_o = scripts.hori(O, od), scripts.vert(O, od)
-    r_o = O.sqrt(x_o * x_o + y_o * y_o)
-    assert_exact(r_g, r_o.val, "sobel + sqrt")
+    m_o = x_o * x_o + y_o * y_o
+    assert_exact(m_g, m_o.val, "sobel, squares and sum")
+    r_o = O.sqrt(m_o)
+    u = _ulps(G.to_array(G.sqrt(G.from_array(m_o.val, bs))), r_o.val)
+    print("  sqrt: max %.2f ulp" % u.max(), flush=True)
+    assert u.max() <= 1.0
     # transcendental nodes on identical FP32 inputs: <= 4 ULP of correctly rounded
     R_in = G.from_array(r_o.val, bs)
     X_in, Y_in = G.from_array(x_o.val, bs), G.from_array(y_o.val, bs)
@@ -305,7 +310,8 @@ def test_sin_cos_pair_within_4ulp(G, O):
     x = np.concatenate(parts).astype(np.float32).reshape(256, 512)
     X = G.from_array(x, (128, 128))
     s, c = G.sin(X), G.cos(X)
-    assert any("sincosf(" in g.get("source", "") for g in G.plan(s, c)["groups"])
+    srcs = [g.get("source", "") for g in G.plan(s, c)["groups"]]
+    assert any("mx::sincos_f32(" in t for t in srcs) or any("sincosf(" in t for t in srcs)
     G.eval(s, c)
     ox = O.from_array(x, (128, 128))
     for name, g, o in [("sin", s, O.sin(ox)), ("cos", c, O.cos(ox))]:
