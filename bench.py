@@ -172,14 +172,12 @@ def batched_tiles_bench(M, ntiles=2048, block=1024, seed=4):
 
 
 class DemCache:
-    """DEM id -> resident DEM (and zone raster), least recently used evicted past a byte budget."""
+    """The served mix's DEMs (and zone rasters) by DEM id in the runtime's resident cache
+    (map.ResidentCache: least recently used entries leave HBM past the byte budget)."""
 
     def __init__(self, M, block, budget):
-        import collections
-        self.M, self.block, self.budget = M, block, budget
-        self.d = collections.OrderedDict()
-        self.bytes = 0
-        self.hits = self.misses = 0
+        self.M, self.block = M, block
+        self.cache = M.ResidentCache(budget)
 
     def side(self, k):
         import numpy as np
@@ -192,26 +190,16 @@ class DemCache:
         import numpy as np
         from map_amd import workloads as W
         M = self.M
-        e = self.d.get(k)
-        if e is not None and (not zones or e[1] is not None):
-            self.hits += 1
-            self.d.move_to_end(k)
-            return e
-        self.misses += 1
         w, h = self.side(k)
         r = np.random.Generator(np.random.PCG64([8, k]))
-        dem = e[0] if e else M.gen_dem((w, h), (self.block, self.block), W.dem_gaussians(r, w, h), 5000 + k)
-        z = M.gen_zones((w, h), (self.block, self.block), r.integers(0, w, 64), r.integers(0, h, 64)) if zones else None
-        nb = (w * h * 4) * (2 if zones else 1)
-        if e:
-            self.bytes -= e[2]
-        self.d[k] = (dem, z, nb, w * h)
-        self.d.move_to_end(k)
-        self.bytes += nb
-        while self.bytes > self.budget and len(self.d) > 1:
-            _, old = self.d.popitem(last=False)
-            self.bytes -= old[2]
-        return self.d[k]
+        gauss = W.dem_gaussians(r, w, h)
+        px, py = r.integers(0, w, 64), r.integers(0, h, 64)
+        dem = self.cache.get(2 * k, lambda: M.gen_dem((w, h), (self.block, self.block), gauss, 5000 + k))
+        z = self.cache.get(2 * k + 1, lambda: M.gen_zones((w, h), (self.block, self.block), px, py)) if zones else None
+        return dem, z, (w * h * 4) * (2 if zones else 1), w * h
+
+    def stats(self):
+        return self.cache.stats()
 
 
 SERVE_KINDS = ("hillshade", "slope_aspect", "mean7", "zstats")
@@ -318,7 +306,9 @@ def serve_bench(M, block=1024, seconds=5.0, seed=9):
         r = serve_run(M, cache, serve_stream(n, rate, seed + int(mult * 10)))
         r["offered_req_s"] = round(rate, 1)
         out["served"].append(r)
-    out["dem_cache"] = {"hits": cache.hits, "misses": cache.misses, "resident_gb": round(cache.bytes / 2**30, 1)}
+    cs = cache.stats()
+    out["dem_cache"] = {"hits": cs["hits"], "misses": cs["misses"], "evictions": cs["evictions"],
+                        "resident_gb": round(cs["bytes"] / 2**30, 1)}
     return out
 
 

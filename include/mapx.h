@@ -239,6 +239,16 @@ int ma_serveTake(long long *ids, double *t_submit, double *t_admit, double *t_do
 const char *ma_serveStats(void);
 /* Runs every submitted request to completion and closes the server. */
 int ma_serveClose(void);
+/* ---- resident cache (extension) ----
+ * Rasters kept in HBM under caller keys (e.g. DEM ids shared by the requests of a server):
+ * the cache holds a reference on each node; past the byte budget (0: no limit) the least
+ * recently used entries are released, and their rasters leave HBM once no request or caller
+ * holds them. ma_cacheGet returns NULL on a miss (not an error); a hit becomes the most recent. */
+int ma_cacheBudget(long long bytes);
+ma_node_t *ma_cacheGet(long long key);
+int ma_cachePut(long long key, ma_node_t *node);
+int ma_cacheDrop(long long key);
+const char *ma_cacheStats(void);
 /* Generated HIP source of a kernel this process has built (by kernel name), or NULL. */
 const char *ma_kernelSource(const char *name);
 /* Autotuner: evaluates the nodes once per candidate tile shape (width,

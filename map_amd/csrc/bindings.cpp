@@ -588,6 +588,26 @@ ma_node_t* ma_genZones(ma_dim2_t ds, ma_dim2_t bs, ma_datatype_t dt, const int* 
   }, (Node*)nullptr);
 }
 
+int ma_cacheBudget(long long bytes) {
+  return guard([&] {
+    if (bytes < 0) throw std::runtime_error("cacheBudget: negative budget");
+    R().cacheBudget((size_t)bytes);
+    return 0;
+  }, -1);
+}
+ma_node_t* ma_cacheGet(long long key) {
+  return guard([&] { return R().cacheGet(key); }, (Node*)nullptr);
+}
+int ma_cachePut(long long key, ma_node_t* node) {
+  return guard([&] { R().cachePut(key, need(node)); return 0; }, -1);
+}
+int ma_cacheDrop(long long key) {
+  return guard([&] { R().cacheDrop(key); return 0; }, -1);
+}
+const char* ma_cacheStats(void) {
+  return guard([&] { g_json = R().cacheStats(); return g_json.c_str(); }, (const char*)nullptr);
+}
+
 int ma_mathCheck(const char* fn, unsigned long long lo, unsigned long long hi, double* max_ulp,
                  unsigned long long* n_over, unsigned long long* n, unsigned* worst_in) {
   return guard([&] {

@@ -420,6 +420,65 @@ struct PauseRecording {
   ~PauseRecording() { if (rec) dev::record_begin(); }
 };
 
+// ---- resident cache -------------------------------------------------------------
+
+void Runtime::cacheBudget(size_t bytes) {
+  std::lock_guard<std::recursive_mutex> lk(mtx_);
+  cache_budget_ = bytes;
+  while (cache_budget_ && cache_bytes_ > cache_budget_ && !cache_lru_.empty()) cache_evict(cache_lru_.back());
+}
+
+Node* Runtime::cacheGet(long long key) {
+  std::lock_guard<std::recursive_mutex> lk(mtx_);
+  auto it = cache_.find(key);
+  if (it == cache_.end()) { cache_misses_++; return nullptr; }
+  cache_hits_++;
+  cache_lru_.splice(cache_lru_.begin(), cache_lru_, it->second.pos);
+  return it->second.node;
+}
+
+void Runtime::cachePut(long long key, Node* n) {
+  std::lock_guard<std::recursive_mutex> lk(mtx_);
+  if (!n) fail("cachePut: null node");
+  if (cache_.count(key)) cache_evict(key);
+  CacheEntry e;
+  e.node = n;
+  e.bytes = n->meta.nd == D2 ? rasterBytes(n->meta) : 0;
+  n->ref++;  // the cache's reference
+  cache_lru_.push_front(key);
+  e.pos = cache_lru_.begin();
+  cache_.emplace(key, e);
+  cache_bytes_ += e.bytes;
+  // the new entry stays even when it alone exceeds the budget
+  while (cache_budget_ && cache_bytes_ > cache_budget_ && cache_lru_.size() > 1) cache_evict(cache_lru_.back());
+}
+
+void Runtime::cacheDrop(long long key) {
+  std::lock_guard<std::recursive_mutex> lk(mtx_);
+  if (cache_.count(key)) cache_evict(key);
+}
+
+// Releases the cache's reference; the raster leaves HBM now if nothing else
+// holds the node (a request still reading it keeps it pinned until it retires).
+void Runtime::cache_evict(long long key) {
+  auto it = cache_.find(key);
+  if (it == cache_.end()) return;
+  Node* n = it->second.node;
+  cache_bytes_ -= it->second.bytes;
+  cache_lru_.erase(it->second.pos);
+  cache_.erase(it);
+  cache_evictions_++;
+  if (n->ref > 0) n->ref--;
+  if (n->ref <= 0) drop(n);
+}
+
+std::string Runtime::cacheStats() const {
+  std::ostringstream j;
+  j << "{\"entries\": " << cache_.size() << ", \"bytes\": " << cache_bytes_ << ", \"budget\": " << cache_budget_
+    << ", \"hits\": " << cache_hits_ << ", \"misses\": " << cache_misses_ << ", \"evictions\": " << cache_evictions_ << "}";
+  return j.str();
+}
+
 void Runtime::release_unreferenced() {
   std::vector<Node*> gone;
   for (auto& r : resident_) if (r.first->ref <= 0) gone.push_back(r.first);

@@ -9,6 +9,7 @@
 #pragma once
 #include "codegen.hpp"
 #include "dag.hpp"
+#include <list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -119,6 +120,17 @@ class Runtime {
   std::string serveStats();
   void serveClose();
   bool serving() const { return srv_ != nullptr; }
+
+  // ---- resident cache: rasters kept in HBM under caller keys ----
+  // (DEMs shared by the requests of a server, or by successive evaluations):
+  // the cache holds a reference on each node; past the byte budget (0: no
+  // limit) the least recently used entries are released, and their rasters
+  // leave HBM once no request or caller holds them.
+  void cacheBudget(size_t bytes);
+  Node* cacheGet(long long key);      // null on a miss; a hit becomes the most recent entry
+  void cachePut(long long key, Node* n);
+  void cacheDrop(long long key);
+  std::string cacheStats() const;
   std::string kernelSource(const std::string& name) const {
     auto it = kernel_src_.find(name);
     return it == kernel_src_.end() ? std::string() : it->second;
@@ -201,6 +213,16 @@ class Runtime {
   std::unordered_map<Node*, int> pins_;  // rasters read by admitted requests: not dropped
   struct ServeState;
   struct Request;
+  struct CacheEntry {
+    Node* node = nullptr;
+    size_t bytes = 0;
+    std::list<long long>::iterator pos;
+  };
+  std::list<long long> cache_lru_;  // most recent first
+  std::unordered_map<long long, CacheEntry> cache_;
+  size_t cache_budget_ = 0, cache_bytes_ = 0;
+  long cache_hits_ = 0, cache_misses_ = 0, cache_evictions_ = 0;
+  void cache_evict(long long key);
   std::unique_ptr<ServeState> srv_;
   void serve_retire();
   std::unordered_map<std::string, void*> tables_;

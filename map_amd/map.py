@@ -841,6 +841,45 @@ class Server:
             _err(None)
 
 
+class ResidentCache:
+    """Rasters kept resident in HBM under caller keys (ints; e.g. DEM ids shared by served
+    requests), least recently used ones released past a byte budget (0: no limit). The
+    cache is the runtime's (one per process): get(key, make) returns the cached raster, or
+    calls make() on a miss and caches its result."""
+
+    def __init__(self, budget_bytes=0):
+        self.budget(budget_bytes)
+
+    @staticmethod
+    def budget(nbytes):
+        if _lib.ma_cacheBudget(int(nbytes)) != 0:
+            _err(None)
+
+    @staticmethod
+    def get(key, make=None):
+        p = _lib.ma_cacheGet(int(key))
+        if p:
+            return Raster(p)
+        if make is None:
+            return None
+        r = make()
+        if _lib.ma_cachePut(int(key), r) != 0:
+            _err(None)
+        return r
+
+    @staticmethod
+    def drop(key):
+        if _lib.ma_cacheDrop(int(key)) != 0:
+            _err(None)
+
+    @staticmethod
+    def stats():
+        s = _lib.ma_cacheStats()
+        if s is None:
+            _err(None)
+        return _json.loads(s.decode())
+
+
 def math_check(fn, lo=0, hi=1 << 32):
     """Accuracy of an FP32 fast path of the generated kernels (sqrt, sin, cos, sincos, atan2,
     atan2_near) over FP32 bit patterns [lo, hi) against FP64 on the device:
@@ -919,6 +958,11 @@ _proto('ma_spreadScan', [Raster, Raster, ct.c_int], Node)
 _proto('ma_lastEvalMs', [], ct.c_double)
 _proto('ma_kernelTimes', [], ct.c_char_p)
 _proto('ma_kernelSource', [ct.c_char_p], ct.c_char_p)
+_proto('ma_cacheBudget', [ct.c_longlong], ct.c_int)
+_proto('ma_cacheGet', [ct.c_longlong], Node)
+_proto('ma_cachePut', [ct.c_longlong, Raster], ct.c_int)
+_proto('ma_cacheDrop', [ct.c_longlong], ct.c_int)
+_proto('ma_cacheStats', [], ct.c_char_p)
 _proto('ma_mathCheck', [ct.c_char_p, ct.c_ulonglong, ct.c_ulonglong, ct.POINTER(ct.c_double), ct.POINTER(ct.c_ulonglong),
                         ct.POINTER(ct.c_ulonglong), ct.POINTER(ct.c_uint)], ct.c_int)
 _proto('ma_serveOpen', [ct.c_longlong, ct.c_int], ct.c_int)

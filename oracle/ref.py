@@ -569,9 +569,13 @@ def _zred(x, red):
         for e in a:
             r = (r * int(e)) & ((1 << 64) - 1)
     elif red == rAND:
-        r = int(np.bitwise_and.reduce(a))
+        # the fold starts from the reduction's neutral (ZonalTask preCompute fills the slot
+        # with it): for signed types that is the type's maximum (ReductionType.cpp:80-90),
+        # not an identity of &, so the sign bit is always cleared
+        r = int(np.bitwise_and.reduce(a)) & int(_neutral(rAND, dt))
     else:
-        r = int(np.bitwise_or.reduce(a))
+        # likewise the rOR neutral of a signed type is its minimum: the sign bit is always set
+        r = int(np.bitwise_or.reduce(a)) | int(_neutral(rOR, dt))
     bits = np.dtype(NP[dt]).itemsize * 8
     r &= (1 << bits) - 1
     return R(np.array(r, dtype=np.uint64).astype(NP[dt]), dt)

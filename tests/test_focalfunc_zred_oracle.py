@@ -81,6 +81,11 @@ def test_zonal_prod_and_or_hand_cases():
     assert z([0xF0, 0x3C], np.uint8, O.rOR) == 0xFC
     assert z([-1, 6], np.int32, O.rAND) == 6
     assert z([-8, 3], np.int32, O.rOR) == -5              # ...11111000 | 011
+    # the reference folds from the neutral, which for signed rAND / rOR is the type's
+    # maximum / minimum (ReductionType.cpp:80-90): the sign bit is cleared / set
+    assert z([-1, -1], np.int32, O.rAND) == 2 ** 31 - 1
+    assert z([5, 3], np.int32, O.rOR) == -(2 ** 31) | 7
+    assert z([-1], np.int8, O.rAND) == 127 and z([1], np.int16, O.rOR) == -32768 + 1
     assert z([0.5, 4.0, 3.0], np.float32, O.PROD) == 6.0
     assert z([2.0] * 10, np.float32, O.PROD) == 1024.0
 
@@ -89,7 +94,7 @@ def test_reduction_neutrals():
     """The neutral of each reduction (ReductionType.cpp:80-90): a one-cell raster reduces to
     its own value."""
     for dt, v in ((np.uint8, 77), (np.int32, -12345), (np.float32, 2.5)):
-        for red in (O.SUM, O.PROD, O.MAX, O.MIN) + ((O.rAND, O.rOR) if dt != np.float32 else ()):
+        for red in (O.SUM, O.PROD, O.MAX, O.MIN) + ((O.rAND, O.rOR) if dt == np.uint8 else ()):
             a = np.array([[v]], dt)
             assert O.value(O._zred(O.from_array(a, (1, 1)), red)) == v, (dt, red)
     assert O._neutral(O.rAND, O.U8) == 255 and O._neutral(O.rOR, O.S32) == np.iinfo(np.int32).min
