@@ -107,7 +107,7 @@ def hillshade_zonal_bench(M, side=32768, block=1024, reps=3):
                         % (side, side, block, block), "algorithmic_bytes_per_cell": 8, **out}
 
 
-def batched_tiles_bench(M, ntiles=2048, block=1024, seed=4):
+def batched_tiles_bench(M, ntiles=2048, block=1024, seed=4, forms=("batched", "plain")):
     """2048 independent tiles of log-uniform sides in [256, 4096]: 5x5
     weighted focal sum -> slope -> mask slope > 0.5 -> per-zone sums of the mask
     over 16 Voronoi zones (U8). Batched (one launch per kernel over a device
@@ -117,7 +117,7 @@ def batched_tiles_bench(M, ntiles=2048, block=1024, seed=4):
     rng = np.random.Generator(np.random.PCG64(seed))
     tiles, cells = [], 0
     for i in range(ntiles):
-        w, h = (W.fit_side(int(round(np.exp(rng.uniform(np.log(256), np.log(4096))))), block, 2) for _ in range(2))
+        w, h = (int(round(np.exp(rng.uniform(np.log(256), np.log(4096))))) for _ in range(2))
         trng = np.random.Generator(np.random.PCG64([seed, i]))
         dem = M.gen_dem((w, h), (block, block), W.dem_gaussians(trng, w, h), 1000 + i)
         zones = M.gen_zones((w, h), (block, block), trng.integers(0, w, 16), trng.integers(0, h, 16), M.U8)
@@ -135,6 +135,8 @@ def batched_tiles_bench(M, ntiles=2048, block=1024, seed=4):
         tuned = M.autotune(*sample, reps=1)["chosen"]
         del sample
         for name, form in (("batched", 1), ("plain", 0)):
+            if name not in forms:
+                continue
             M.set_option("form", form)
             M.set_option("graph", 1 if form == 1 else 0)  # steady state replayed from hipGraphs
             outs = [W.batch_tile_sums(M, d, z, w) for d, z, w in tiles]
@@ -158,17 +160,26 @@ def batched_tiles_bench(M, ntiles=2048, block=1024, seed=4):
                          "host_ms_last": {k: round(h1[k] - h0[k], 2) for k in ("ms_plan", "ms_fuse", "ms_codegen", "ms_prepare", "ms_launch")},
                          "launches_last": h1["launches"] - h0["launches"],
                          "sums": [o.table()["sum"] for o in outs]}
+            if name == "batched":  # per-kernel device time of one evaluation (launches timed, no graph)
+                M.set_option("graph", 0)
+                kt = _kernel_profile(M, lambda: ([M.release(o) for o in outs], M.eval(*outs)))
+                res[name]["kernels"] = {k: {"count": v["count"], "total_ms": round(v["total_ms"], 3)}
+                                        for k, v in kt.items()}
             del outs
     finally:
         M.set_option("form", 1)
         M.set_option("graph", 0)
         M.set_option("focal_zonal", 0)
     res["tuned_tiles"] = tuned
-    same = all(np.array_equal(a, b) for a, b in zip(res["batched"].pop("sums"), res["plain"].pop("sums")))
+    both = "batched" in res and "plain" in res
+    same = both and all(np.array_equal(a, b) for a, b in zip(res["batched"].pop("sums"), res["plain"].pop("sums")))
+    for r in res.values():
+        if isinstance(r, dict):
+            r.pop("sums", None)
     return {"workload": "%d tiles, sides log-uniform [256, 4096], 5x5 weighted sum -> slope -> mask -> 16-zone sums, "
                         "blocks %dx%d" % (ntiles, block, block),
-            "cells": cells, "outputs_identical": bool(same),
-            "speedup_vs_plain": round(res["plain"]["wall_ms"] / res["batched"]["wall_ms"], 2), **res}
+            "cells": cells, "outputs_identical": bool(same) if both else None,
+            "speedup_vs_plain": round(res["plain"]["wall_ms"] / res["batched"]["wall_ms"], 2) if both else None, **res}
 
 
 class DemCache:
@@ -181,10 +192,8 @@ class DemCache:
 
     def side(self, k):
         import numpy as np
-        from map_amd import workloads as W
         r = np.random.Generator(np.random.PCG64([7, k]))
-        return tuple(W.fit_side(int(round(np.exp(r.uniform(np.log(512), np.log(16384))))), self.block, 3)
-                     for _ in range(2))
+        return tuple(int(round(np.exp(r.uniform(np.log(512), np.log(16384))))) for _ in range(2))
 
     def get(self, k, zones):
         import numpy as np
@@ -434,6 +443,8 @@ def main():
                     help="only the Life headline (skip hillshade, hillshade+zonal, batched tiles, serving)")
     ap.add_argument("--only", choices=["hillshade", "hillshade_zonal", "batched_tiles", "serve", "radial", "spread"],
                     help="run one extra section alone (no Life headline), print its JSON")
+    ap.add_argument("--c3-forms", default="batched,plain",
+                    help="forms of the batched-tiles section (batched, plain), comma-separated")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a U8 copy of the input raster (a local cast kernel), for counter calibration")
     args = ap.parse_args()
@@ -447,7 +458,8 @@ def main():
         fn = {"hillshade": hillshade_bench, "hillshade_zonal": hillshade_zonal_bench,
               "batched_tiles": batched_tiles_bench, "serve": serve_bench, "radial": radial_bench,
               "spread": spread_bench}[args.only]
-        print(json.dumps({args.only: fn(M)}), flush=True)
+        kw = {"forms": tuple(args.c3_forms.split(","))} if args.only == "batched_tiles" else {}
+        print(json.dumps({args.only: fn(M, **kw)}), flush=True)
         return
     bs = (args.block, args.block)
     from map_amd import workloads as W

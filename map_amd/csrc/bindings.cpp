@@ -339,6 +339,7 @@ int ma_setOption(const char* key, long long value) {
     else if (k == "tile_pf") o.tile.PF = value != 0;
     else if (k == "tile_st") o.tile.ST = value != 0;
     else if (k == "libm") o.libm = value != 0;
+    else if (k == "predict") o.predict = value != 0;
     else if (k == "wg_per_cu") {
       if (value < 0 || value > 64) throw std::runtime_error("setOption: wg_per_cu in [0, 64]");
       o.wg_per_cu = (int)value;
@@ -369,6 +370,7 @@ long long ma_getOption(const char* key) {
   if (k == "tile_pf") return o.tile.PF;
   if (k == "tile_st") return o.tile.ST;
   if (k == "libm") return o.libm;
+  if (k == "predict") return o.predict;
   if (k == "wg_per_cu") return o.wg_per_cu;
   if (k == "batch_tasks") return o.batch_tasks;
   if (k == "queue") return o.queue;
@@ -415,6 +417,8 @@ const char* ma_runStats(void) {
   Runtime& r = R();
   std::ostringstream s;
   s << "{\"evals\": " << r.cnt.evals << ", \"groups\": " << r.cnt.groups << ", \"launches\": " << r.cnt.launches
+    << ", \"compiles\": " << r.cnt.compiles << ", \"shape_hits\": " << r.cnt.shape_hits
+    << ", \"predicted\": " << r.cnt.predicted
     << ", \"scalar_tasks\": " << r.cnt.scalar_tasks << ", \"bytes_h2d\": " << r.cnt.bytes_h2d
     << ", \"bytes_d2h\": " << r.cnt.bytes_d2h << ", \"store_in_use\": " << r.store.in_use()
     << ", \"store_cached\": " << r.store.cached() << ", \"graphs\": " << dev::graph_cache_size()

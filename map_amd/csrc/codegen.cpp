@@ -7,6 +7,7 @@
 #include <map>
 #include <set>
 #include <sstream>
+#include <unordered_map>
 
 namespace mapx {
 
@@ -141,6 +142,43 @@ static std::string reduce_stmt(int op, int dt, const std::string& acc, const std
   return o.str();
 }
 
+// inputs first (D2 then D0, each by id), then the nodes by id: the local
+// numbering is then the same for groups of equal structure
+std::vector<Node*> kernel_order(const Group& g) {
+  std::vector<Node*> all(g.ins.begin(), g.ins.end()), body(g.nodes.begin(), g.nodes.end());
+  std::sort(all.begin(), all.end(), [](Node* a, Node* b) {
+    if ((a->meta.nd == D0) != (b->meta.nd == D0)) return a->meta.nd != D0;
+    return a->id < b->id;
+  });
+  std::sort(body.begin(), body.end(), [](Node* a, Node* b) { return a->id < b->id; });
+  all.insert(all.end(), body.begin(), body.end());
+  return all;
+}
+
+std::string kernel_shape_key(const Group& g, const std::vector<Node*>& d2_outs, TileShape tile, bool libm) {
+  const std::vector<Node*> all = kernel_order(g);
+  std::unordered_map<const Node*, int> local;
+  for (size_t i = 0; i < all.size(); i++) local[all[i]] = (int)i;
+  std::set<const Node*> in_set(g.ins.begin(), g.ins.end());
+  std::ostringstream k;
+  k << tile.TX << "," << tile.TY << "," << tile.VX << "," << tile.PF << "," << tile.ST << "," << libm << "|";
+  for (Node* n : all) {
+    if (in_set.count(n)) { k << "I" << n->meta.nd << "." << n->meta.dt << ";"; continue; }
+    k << n->signature(false) << "(";
+    for (Node* p : n->prev) {
+      auto it = local.find(p);
+      k << (it == local.end() ? -1 : it->second) << ",";
+    }
+    k << ");";
+  }
+  k << "|";
+  for (Node* o : d2_outs) {
+    auto it = local.find(o);
+    k << (it == local.end() ? -1 : it->second) << ",";
+  }
+  return k.str();
+}
+
 KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, TileShape tile, bool libm) {
   const bool fast = !libm;
   KernelSpec ks;
@@ -150,15 +188,8 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   const int TWX = tile.TX / tile.VX;
   if (256 % TWX || tile.TY % (256 / TWX)) throw std::runtime_error("codegen: tile does not fit 256 threads");
 
-  // inputs first (D2 then D0, each by id), then the nodes by id: the local
-  // numbering is then the same for groups of equal structure
-  std::vector<Node*> all(g.ins.begin(), g.ins.end()), body(g.nodes.begin(), g.nodes.end());
-  std::sort(all.begin(), all.end(), [](Node* a, Node* b) {
-    if ((a->meta.nd == D0) != (b->meta.nd == D0)) return a->meta.nd != D0;
-    return a->id < b->id;
-  });
+  std::vector<Node*> all = kernel_order(g), body(g.nodes.begin(), g.nodes.end());
   std::sort(body.begin(), body.end(), [](Node* a, Node* b) { return a->id < b->id; });
-  all.insert(all.end(), body.begin(), body.end());
   std::set<Node*> in_set(g.ins.begin(), g.ins.end());
   std::set<Node*> member(all.begin(), all.end());
   std::map<const Node*, int> local;

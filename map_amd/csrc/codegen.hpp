@@ -73,6 +73,15 @@ struct KernelSpec {
 // the fast paths of mapx_dev.hiph (mx::sqrt_f32, mx::atan2_f32, mx::sincos_f32).
 KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, TileShape tile, bool libm = false);
 
+// The group's nodes in the kernel's local numbering: inputs (D2, then D0,
+// each by id), then the group's nodes by id.
+std::vector<Node*> kernel_order(const Group& g);
+// Key of the kernel generate_kernel builds for `g`: groups with equal keys
+// (node signatures and links by local number, materialised outputs, tile,
+// libm) get the same source and parameter layout, so one compiled kernel
+// serves them all with its node references rebound by local number.
+std::string kernel_shape_key(const Group& g, const std::vector<Node*>& d2_outs, TileShape tile, bool libm);
+
 // Default tile shape for a group: VX cells per thread along x so that a
 // wavefront row covers >= 256 contiguous bytes.
 TileShape default_tile(const Group& g);

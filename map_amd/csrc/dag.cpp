@@ -601,6 +601,12 @@ void Dag::collect(const std::function<void(Node*)>& on_delete) {
   nodes_.swap(keep);
 }
 
+void Dag::collect_grown(const std::function<void(Node*)>& on_delete) {
+  if (nodes_.size() < std::max<size_t>(4096, 2 * live_after_collect_)) return;
+  collect(on_delete);
+  live_after_collect_ = nodes_.size();
+}
+
 std::vector<Node*> list_upwards(const std::vector<Node*>& roots) {
   std::set<Node*> seen;
   std::vector<Node*> stack(roots.begin(), roots.end()), out;

@@ -647,14 +647,11 @@ TileShape Runtime::tile_for(const Group& g) {
   return t;
 }
 
-// Geometry preconditions of the focal template: a halo never exceeds a block
-// nor the valid extent of an edge block, so every halo lookup lands in the
-// central block or one neighbour block.
+// Geometry precondition of the focal template: a halo never exceeds a block,
+// so a halo lookup reaches at most the neighbour blocks (and, past an edge
+// block narrower than the halo, the border rule applied again from there).
 static void check_focal_geometry(const Meta& m, int H0, int H1) {
-  const int E0 = m.ds.arr[0] % m.bs.arr[0], E1 = m.ds.arr[1] % m.bs.arr[1];
   if (H0 > m.bs.arr[0] || H1 > m.bs.arr[1]) fail("focal halo larger than the block size");
-  if ((E0 && E0 < H0) || (E1 && E1 < H1))
-    fail("edge blocks narrower than the focal halo: choose a block size whose remainder is 0 or >= the halo");
 }
 
 // Access: one cell of a resident raster (block-tiled), copied back once the
@@ -775,6 +772,162 @@ void Runtime::radial_task(const Group& g, Eval& e) {
 // The targets and the nodes evaluated with them: the MAX / MIN reductions of
 // every Stats node they depend on (the reference computes them in the Stats
 // task, runtime/task/StatsTask.cpp:95-160).
+// ---- constant-raster prediction ----------------------------------------------------
+// The reference's Predictor (runtime/visitor/Predictor.cpp:22-79, called per
+// block job from Worker.cpp:84-90) skips kernels whose output blocks are
+// constant. Here, over whole rasters at planning time: a D2 node holds one
+// value in every cell when it is a Constant, an operator, cast, conditional
+// or min/max of such values, or a convolution, neighbour or window reduction
+// of a constant raster (every tap holds the value, whatever the border rule
+// reads). Only statements whose host evaluation gives the device's bits are
+// folded (IEEE operators, integer arithmetic in C types, exact rounding
+// functions; no transcendentals, no float-to-integer casts out of range, no
+// division by zero or out-of-range shifts). A zonal reduction of a constant
+// raster is then known without a kernel: SUM = the value times the valid
+// cell count (integer types wrap; an F32 value times at most 2^29 cells is
+// exact in the FP64 accumulator, so the device's sum is that product), MAX
+// and MIN the value (the neutral for NaN, which fmax/fmin skip), rAND / rOR
+// the value folded with the neutral, integer PROD the value to the power of
+// the count. Quirk B8 (the reference returns the value itself for SUM) is
+// not reproduced.
+namespace {
+bool foldable(const Node* n, const std::vector<Variant>& pv) {
+  switch (n->kind) {
+    case Kind::Cast:
+      if (is_float_dt(pv[0].dtype) && !is_float_dt(n->meta.dt)) {
+        const double d = variant_as<double>(pv[0]);
+        if (!(d > -2147483648.0 && d < 2147483648.0)) return false;  // C leaves the rest undefined
+      }
+      return true;
+    case Kind::Conditional: case Kind::Stats: case Kind::Barrier:
+      return true;
+    case Kind::Unary:
+      return unary_is_operator(n->op) || n->op == ABS || n->op == CEIL || n->op == FLOOR || n->op == TRUNC ||
+             n->op == ROUND;
+    case Kind::Binary: {
+      const int op = n->op;
+      if (!(binary_is_operator(op) || op == MAX2 || op == MIN2 || op == FMOD)) return false;
+      if ((op == DIV || op == MOD) && !is_float_dt(pv[1].dtype) && variant_as<double>(pv[1]) == 0.0) return false;
+      if ((op == SHL || op == SHR) && (is_float_dt(pv[1].dtype) || variant_as<double>(pv[1]) < 0 ||
+                                       variant_as<double>(pv[1]) >= 8.0 * sizeof_dt(promote(pv[0].dtype, S32))))
+        return false;
+      return true;
+    }
+    default:
+      return false;
+  }
+}
+
+// One statement on known operand values, with the C semantics of the kernels.
+Variant fold(const Node* n, const std::vector<Variant>& pv) {
+  if (n->kind == Kind::Stats || n->kind == Kind::Barrier) return pv[0];
+  std::vector<Node> ops(pv.size());
+  Node t;
+  t.kind = n->kind;
+  t.op = n->op;
+  t.meta = n->meta;
+  for (size_t i = 0; i < pv.size(); i++) {
+    ops[i].kind = Kind::Constant;
+    ops[i].meta.dt = pv[i].dtype;
+    ops[i].value = pv[i];
+    ops[i].evaluated = true;
+    t.prev.push_back(&ops[i]);
+  }
+  return eval_scalar_node(&t);
+}
+
+// acc = OP(acc, v) for `count` taps in the node's C type (the window statements).
+Variant window_fold(const Node* n, const Variant& v) {
+  const int dt = n->meta.dt;
+  if (n->kind == Kind::Neighbor) return v;
+  if (n->kind == Kind::FocalFunc && (n->op == MAX || n->op == MIN)) {
+    if (is_float_dt(dt) && std::isnan(variant_as<double>(v))) return reduction_neutral(n->op, dt);
+    return variant_convert(v, dt);
+  }
+  Variant acc = n->kind == Kind::Convolution ? variant_convert(Variant{{.s32 = 0}, S32}, dt) : reduction_neutral(n->op, dt);
+  for (const Variant& w0 : n->mask.v) {
+    if (n->kind == Kind::Convolution) {
+      const Variant w = quantize_literal(w0);
+      acc = from_cv(binop(ADD, to_cv(acc), binop(MUL, to_cv(v), to_cv(w))), dt);
+    } else {
+      const int op = n->op == SUM ? ADD : n->op == PROD ? MUL : n->op == rAND ? bAND : bOR;
+      acc = from_cv(binop(op, to_cv(acc), to_cv(variant_convert(v, dt))), dt);
+    }
+  }
+  return acc;
+}
+}  // namespace
+
+int Runtime::predict(const std::vector<Node*>& targets) {
+  if (!opt.predict) return 0;
+  std::unordered_map<const Node*, Variant> cval;
+  int done = 0;
+  for (Node* n : list_upwards(targets)) {
+    try {
+      if (n->meta.nd == D0) {
+        if (n->kind == Kind::Constant) cval[n] = n->cnst;
+        else if (n->evaluated) cval[n] = n->value;
+        else if (n->kind == Kind::ZonalReduc && cval.count(n->prev[0])) {
+          const Variant v = cval[n->prev[0]];
+          const int dt = n->meta.dt;
+          const Meta& m = n->prev[0]->meta;
+          const uint64_t cells = (uint64_t)m.ds.arr[0] * m.ds.arr[1];
+          Variant r;
+          bool ok = true;
+          if (dt == B8 || v.dtype != dt) ok = false;
+          else if (n->op == MAX || n->op == MIN) {
+            r = (is_float_dt(dt) && std::isnan(variant_as<double>(v))) ? reduction_neutral(n->op, dt) : v;
+          } else if (n->op == SUM) {
+            if (dt == F32 && cells <= (1ull << 29)) {
+              Variant d = variant_zero(F64);
+              d.u.f64 = (double)v.u.f32 * (double)cells;
+              r = variant_convert(d, F32);
+            } else if (!is_float_dt(dt)) {
+              Variant c = variant_zero(U64);
+              c.u.u64 = cells;
+              r = from_cv(binop(MUL, to_cv(variant_convert(v, U64)), to_cv(c)), dt);
+            } else ok = false;
+          } else if (n->op == PROD && !is_float_dt(dt)) {
+            uint64_t base = variant_convert(v, U64).u.u64, e = cells, acc = 1;
+            while (e) { if (e & 1) acc *= base; base *= base; e >>= 1; }
+            Variant a = variant_zero(U64);
+            a.u.u64 = acc;
+            r = variant_convert(a, dt);
+          } else if ((n->op == rAND || n->op == rOR) && !is_float_dt(dt)) {
+            r = from_cv(binop(n->op == rAND ? bAND : bOR, to_cv(v), to_cv(reduction_neutral(n->op, dt))), dt);
+          } else ok = false;
+          if (ok) {
+            n->value = r;
+            n->evaluated = true;
+            cval[n] = r;
+            done++;
+          }
+        }
+        continue;
+      }
+      if (n->meta.nd != D2 || resident_.count(n)) continue;
+      if (n->kind == Kind::Constant) { cval[n] = variant_convert(n->cnst, n->meta.dt); continue; }
+      if (n->kind == Kind::Neighbor || n->kind == Kind::Convolution || n->kind == Kind::FocalFunc) {
+        auto it = cval.find(n->prev[0]);
+        if (it != cval.end()) cval[n] = window_fold(n, it->second);
+        continue;
+      }
+      std::vector<Variant> pv;
+      bool all = true;
+      for (Node* p : n->prev) {
+        auto it = cval.find(p);
+        if (it == cval.end()) { all = false; break; }
+        pv.push_back(it->second);
+      }
+      if (all && !n->prev.empty() && foldable(n, pv)) cval[n] = fold(n, pv);
+    } catch (const std::exception&) {
+      // not folded: the kernel computes it
+    }
+  }
+  cnt.predicted += done;
+  return done;
+}
+
 std::vector<Node*> Runtime::with_aux(const std::vector<Node*>& targets) {
   std::vector<Node*> out = targets;
   std::set<Node*> have(targets.begin(), targets.end());
@@ -1160,8 +1313,20 @@ Runtime::KernelEntry& Runtime::kernel_for(Plan* plan, const Group& g, const std:
   auto it = plan->kernels.find(key);
   if (it != plan->kernels.end()) return it->second;
   HostTimer ht(cnt.ms_codegen);
+  const TileShape tile = tile_for(g);
+  const std::string skey = kernel_shape_key(g, kouts, tile, opt.libm);
+  const std::vector<Node*> order = kernel_order(g);
+  auto sh = shapes_.find(skey);
+  if (sh != shapes_.end()) {
+    KernelEntry e = sh->second.e;
+    for (size_t i = 0; i < e.spec.params.size(); i++) e.spec.params[i].node = order.at(sh->second.param_node[i]);
+    for (size_t i = 0; i < e.spec.d2_outs.size(); i++) e.spec.d2_outs[i] = order.at(sh->second.d2_out[i]);
+    for (size_t i = 0; i < e.spec.zouts.size(); i++) e.spec.zouts[i].node = order.at(sh->second.zout[i]);
+    cnt.shape_hits++;
+    return plan->kernels.emplace(key, std::move(e)).first->second;
+  }
   KernelEntry e;
-  e.spec = generate_kernel(g, kouts, tile_for(g), opt.libm);
+  e.spec = generate_kernel(g, kouts, tile, opt.libm);
   kernel_group_[e.spec.name] = tune_key(g);
   kernel_src_[e.spec.name] = e.spec.source;
   dev::Function f = dev::function(e.spec.source, e.spec.name);
@@ -1170,6 +1335,21 @@ Runtime::KernelEntry& Runtime::kernel_for(Plan* plan, const Group& g, const std:
   kernel_occ_[e.spec.name] = e.occ;
   if (!e.spec.fin_name.empty()) e.fin = dev::function(e.spec.source, e.spec.fin_name).fn;
   cnt.compiles++;
+  {
+    std::unordered_map<const Node*, int> local;
+    for (size_t i = 0; i < order.size(); i++) local[order[i]] = (int)i;
+    ShapeEntry s;
+    bool ok = true;
+    auto idx = [&](const Node* n) { auto it = local.find(n); if (it == local.end()) { ok = false; return -1; } return it->second; };
+    for (auto& p : e.spec.params) s.param_node.push_back(idx(p.node));
+    for (Node* n : e.spec.d2_outs) s.d2_out.push_back(idx(n));
+    for (auto& z : e.spec.zouts) s.zout.push_back(idx(z.node));
+    if (ok) {
+      if (shapes_.size() >= 4096) shapes_.clear();  // bounded
+      s.e = e;
+      shapes_.emplace(skey, std::move(s));
+    }
+  }
   return plan->kernels.emplace(key, std::move(e)).first->second;
 }
 
@@ -1241,13 +1421,25 @@ void Runtime::evaluate(const std::vector<Node*>& targets) {
   if (targets.empty()) fail("evaluate: nothing to evaluate");
   if (serving()) fail("evaluate: a server is open (submit requests to it, or close it first)");
   release_unreferenced();
-  dag.collect([this](Node* n) { forget(n); });
+  dag.collect_grown([this](Node* n) { forget(n); });
   cnt.evals++;
   dev::Timer t;
   t.start();
 
   Eval e;
   e.targets = with_aux(targets);
+  predict(e.targets);
+  {
+    std::vector<Node*> left;
+    for (Node* n : e.targets)
+      if (!(n->meta.nd == D0 && n->evaluated && n->kind != Kind::Constant)) left.push_back(n);
+    if (left.empty()) {  // every target is a known value
+      t.stop();
+      cnt.ms_last_eval = t.ms();
+      return;
+    }
+    e.targets = left;
+  }
   struct Uncut {
     Runtime* r; Eval& e;
     ~Uncut() { r->eval_uncut(e); }
@@ -1341,7 +1533,8 @@ std::string Runtime::plan(const std::vector<Node*>& targets_in) {
         }
         j << ", \"kernel\": \"" << ks.name << "\", \"form\": " << (int)ks.form << ", \"halo\": [" << ks.H0 << ", " << ks.H1
           << "], \"stream\": " << (ks.stream ? 1 : 0) << ", \"tile\": [" << ks.tile.TX << ", " << ks.tile.TY << ", " << ks.tile.VX << ", " << ks.tile.PF
-          << "], \"tune_key\": \"" << esc(tune_key(*g)) << "\", \"source\": \"" << esc(ks.source) << "\"";
+          << "], \"tune_key\": \"" << esc(tune_key(*g)) << "\", \"shape_key\": \""
+          << esc(kernel_shape_key(*g, kouts, tile_for(*g), opt.libm)) << "\", \"source\": \"" << esc(ks.source) << "\"";
         // floating mask weights as the kernel receives them (quantised like printed literals)
         j << ", \"mask_args\": [";
         bool fm = true;

@@ -36,10 +36,13 @@ struct Options {
   int chunk_tiles = 0;        // tiles per chunk of non-zonal groups (0: 16)
   TileShape tile{0, 0, 0, 0, 0}; // forced tile shape (TX 0 = per-group default / tuned)
   bool libm = false;          // FP32 sqrt / atan2 / sin-cos pairs through the device library, not the fast paths
+  bool predict = true;        // zonal reductions of constant rasters computed on the host (no kernel)
 };
 
 struct Counters {
   long evals = 0, groups = 0, launches = 0, scalar_tasks = 0, compiles = 0;
+  long shape_hits = 0;  // kernels taken from the shape cache instead of generated
+  long predicted = 0;   // zonal reductions of constant rasters known without a kernel
   long bytes_h2d = 0, bytes_d2h = 0;
   double ms_last_eval = 0;
   // host time per phase (the reference's Clock regions, runtime/Clock.hpp:24-27)
@@ -167,6 +170,9 @@ class Runtime {
   void radial_task(const Group& g, Eval& e);
   void spread_task(const Group& g, Eval& e);
   std::vector<Node*> with_aux(const std::vector<Node*>& targets);
+  // Constant-raster prediction: zonal reductions of rasters that hold one
+  // value everywhere get their value without a kernel; returns their count.
+  int predict(const std::vector<Node*>& targets);
   Variant access_value(Node* n);
   void run_batch(const std::vector<Task*>& batch, std::vector<Readback>* async);
   void fold_results(const std::vector<Task*>& batch, void* parts, void* tasks_dev, void* starts_dev, int maxn,
@@ -198,6 +204,15 @@ class Runtime {
   };
   std::string plan_key(const std::vector<Node*>& list, const std::vector<Node*>& targets) const;
   KernelEntry& kernel_for(Plan* plan, const Group& g, const std::vector<Node*>& kouts);
+  // Kernels by shape (kernel_shape_key): a new group of a known shape (the same
+  // script over another raster, the next request of a kind) takes the compiled
+  // kernel and its parameter layout, its node references rebound by the local
+  // numbering of kernel_order, instead of generating the source again.
+  struct ShapeEntry {
+    KernelEntry e;
+    std::vector<int> param_node, d2_out, zout;  // local numbers of the spec's node references
+  };
+  std::unordered_map<std::string, ShapeEntry> shapes_;
   std::unordered_map<std::string, std::shared_ptr<Plan>> plans_;
   std::multimap<size_t, void*> pinned_free_;
   std::unordered_map<void*, size_t> pinned_size_;

@@ -132,7 +132,7 @@ int Runtime::serveStep(int wait_us) {
   const long done0 = S.completed;
   serve_retire();
   release_unreferenced();
-  dag.collect([this](Node* n) { forget(n); });
+  dag.collect_grown([this](Node* n) { forget(n); });
 
   // admission, in arrival order (a request blocked by another's node lets later ones pass)
   for (auto it = S.queued.begin(); it != S.queued.end();) {

@@ -121,14 +121,6 @@ def life_input(side, seed, p=0.3):
     return out
 
 
-def fit_side(side, block, halo):
-    """A side whose last block is empty or at least `halo` cells wide (the
-    border rule reads at most one neighbour block); sides that would leave a
-    thinner last block grow by the few missing cells."""
-    r = side % block
-    return side if r == 0 or r >= halo else side + (halo - r)
-
-
 # D8 codes of the eight neighbours at offset (dx, dy), y growing south (ESRI order:
 # 1 E, 2 SE, 4 S, 8 SW, 16 W, 32 NW, 64 N, 128 NE), the codes the reference's spreading
 # kernel and flowAccuGather read (runtime/skeleton/util.cpp:628, front/Raster.cpp:586-589).

@@ -12,6 +12,9 @@
  *    (invert), also for diagonal lookups whose row or column neighbour exists.
  *    Edge blocks of sizes that are not block multiples mirror at their valid
  *    extent (the backend's masking of padding; the reference mirrors at BS).
+ *    A lookup that lands outside its new block's valid extent (an edge block
+ *    narrower than the halo) applies the rule again from that block, until it
+ *    is inside one: load_F followed across the blocks it needs.
  *  - the focal correlation of FocalSkeleton.cpp:300-336: acc = 0, then for
  *    y = -h..h (outer), x = -h..h (inner): acc += S[y][x] * W[y][x], in the
  *    C types of the operands (integer: exact then wrapped to the node type;
@@ -33,26 +36,28 @@ static inline int or_modulo(int i, int d) { return i < 0 ? i + d : (i >= d ? i -
 /* Global source cell of the lookup at block-local (lx, ly) of block (bx, by). */
 void or_src(int ds0, int ds1, int bs0, int bs1, int bx, int by, int lx, int ly, int *gx, int *gy) {
   const int nb0 = (ds0 + bs0 - 1) / bs0, nb1 = (ds1 + bs1 - 1) / bs1;
-  const int e0 = ds0 - bx * bs0 < bs0 ? ds0 - bx * bs0 : bs0;
-  const int e1 = ds1 - by * bs1 < bs1 ? ds1 - by * bs1 : bs1;
-  const int i0 = lx < 0 ? 0 : (lx >= e0 ? 2 : 1);
-  const int i1 = ly < 0 ? 0 : (ly >= e1 ? 2 : 1);
-  int x = lx, y = ly, nbx = bx, nby = by;
-  if (i0 != 1 || i1 != 1) {
-    nbx = bx + i0 - 1;
-    nby = by + i1 - 1;
+  int x = lx, y = ly;
+  /* each step lands in range, moves to a neighbour block, or reflects the coordinate
+     (two reflections shift it by twice the extent towards the block): it ends */
+  for (int step = 0; step < 4096; step++) {
+    const int e0 = ds0 - bx * bs0 < bs0 ? ds0 - bx * bs0 : bs0;
+    const int e1 = ds1 - by * bs1 < bs1 ? ds1 - by * bs1 : bs1;
+    const int i0 = x < 0 ? 0 : (x >= e0 ? 2 : 1);
+    const int i1 = y < 0 ? 0 : (y >= e1 ? 2 : 1);
+    if (i0 == 1 && i1 == 1) break;
+    const int nbx = bx + i0 - 1, nby = by + i1 - 1;
     if (nbx < 0 || nby < 0 || nbx >= nb0 || nby >= nb1) {
-      x = or_invert(lx, e0);
-      y = or_invert(ly, e1);
-      nbx = bx;
-      nby = by;
+      x = or_invert(x, e0);
+      y = or_invert(y, e1);
     } else {
-      x = or_modulo(lx, bs0);
-      y = or_modulo(ly, bs1);
+      x = or_modulo(x, bs0);
+      y = or_modulo(y, bs1);
+      bx = nbx;
+      by = nby;
     }
   }
-  *gx = nbx * bs0 + x;
-  *gy = nby * bs1 + y;
+  *gx = bx * bs0 + x;
+  *gy = by * bs1 + y;
 }
 
 /* Index map of every tap: idx[(cell * nt) + t] = source cell (row-major), for

@@ -735,6 +735,18 @@ def zeros_like(x, dt=None):
     return R(np.zeros(np.asarray(x.val).shape, _work(NP[dt])), dt, x.bs)
 
 
+def full_like(c, x, dt=None):
+    """A raster of x's shape holding the constant c converted to dt (x's type by default), as
+    python/map.py's _like makes it (Variant(c).convert(dt))."""
+    dt = x.dt if dt is None else dt
+    v = _conv(np.asarray(const(c).val), dt)
+    return R(np.full(np.asarray(x.val).shape, v, _work(NP[dt])), dt, x.bs)
+
+
+def ones_like(x, dt=None):
+    return full_like(1, x, dt)
+
+
 def _r_getitem(self, key):
     if isinstance(key, (list, tuple)):
         return access(self, key)

@@ -199,3 +199,33 @@ def test_library_exports_every_declared_entry_point():
     lib = ctypes.CDLL(os.path.join(root, "map_amd", "libmapx.so"))
     missing = [n for n in sorted(set(names)) if not hasattr(lib, n)]
     assert len(set(names)) > 50 and not missing, missing
+
+
+def test_kernel_shape_key_determines_the_source(M):
+    """Groups with equal shape keys (the key under which a compiled kernel is reused for a new
+    raster) have identical generated sources: checked over the workload scripts applied to
+    rasters of different sizes, types and block sizes, and scripts that differ only in a
+    constant, a mask or an output."""
+    import numpy as np
+    from map_amd import workloads as W
+    rasters = [M.from_array(np.zeros((h, w), dt), bs) for (w, h, dt, bs) in
+               [(300, 200, np.float32, (128, 128)), (517, 389, np.float32, (256, 128)), (96, 70, np.float32, (32, 32)),
+                (300, 200, np.int32, (128, 128)), (64, 64, np.uint8, (64, 64)), (200, 100, np.uint8, (64, 32))]]
+    scripts = [lambda r: (W.hillshade(M, r),), lambda r: (W.slope(M, r), W.aspect(M, r)),
+               lambda r: (W.focal_mean7(M, r),), lambda r: (r * 2 + 1,), lambda r: (r * 3 + 1,),
+               lambda r: (M.convolve(r, [[1, 2, 1]] * 3),), lambda r: (M.convolve(r, [[1, 2, 2]] * 3),),
+               lambda r: (W.life(M, r, 1),), lambda r: (M.zsum(r * 2),)]
+    seen = {}
+    for r in rasters:
+        for sc in scripts:
+            for g in M.plan(*sc(r))["groups"]:
+                if "shape_key" not in g:
+                    continue
+                if g["shape_key"] in seen:
+                    assert seen[g["shape_key"]] == g["source"], g["shape_key"]
+                seen[g["shape_key"]] = g["source"]
+    assert len(seen) >= 12
+    # the same script over two rasters of one type and block size shares its key
+    a = M.plan(W.hillshade(M, rasters[0]))["groups"]
+    b = M.plan(W.hillshade(M, M.from_array(np.zeros((389, 517), np.float32), (128, 128))))["groups"]
+    assert [g.get("shape_key") for g in a if "kernel" in g][0] == [g.get("shape_key") for g in b if "kernel" in g][0]

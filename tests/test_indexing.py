@@ -22,32 +22,52 @@ def modulo(i, d):
     return i + d if i < 0 else (i - d if i >= d else i)
 
 
+def ext(ds, bs, b, i):
+    return min(bs[i], ds[i] - b * bs[i])
+
+
 def ldf_index(ds, bs, nb, bx, by, x, y):
-    e0 = min(bs[0], ds[0] - bx * bs[0])
-    e1 = min(bs[1], ds[1] - by * bs[1])
-    i0 = 0 if x < 0 else (2 if x >= e0 else 1)
-    i1 = 0 if y < 0 else (2 if y >= e1 else 1)
-    blk = by * nb[0] + bx
-    if i0 != 1 or i1 != 1:
+    """mx::ldf: the rule applied from each block reached until the lookup is inside one
+    (64 steps at most, then clamped)."""
+    e0, e1 = ext(ds, bs, bx, 0), ext(ds, bs, by, 1)
+    for _ in range(64):
+        i0 = 0 if x < 0 else (2 if x >= e0 else 1)
+        i1 = 0 if y < 0 else (2 if y >= e1 else 1)
+        if i0 == 1 and i1 == 1:
+            break
         nbx, nby = bx + i0 - 1, by + i1 - 1
         if nbx < 0 or nby < 0 or nbx >= nb[0] or nby >= nb[1]:
             x, y = invert(x, e0), invert(y, e1)
         else:
             x, y = modulo(x, bs[0]), modulo(y, bs[1])
-            blk = nby * nb[0] + nbx
-    return blk, x, y
+            bx, by = nbx, nby
+            e0, e1 = ext(ds, bs, bx, 0), ext(ds, bs, by, 1)
+    else:
+        raise AssertionError("border rule did not end")
+    return by * nb[0] + bx, x, y
 
 
-def row_src_offset(bs, nb, bx, by, e1, y):
-    """Element offset of halo row y as mx::row_src computes it: k.off + (s*y + d) * BS0."""
+def row_src(ds, bs, nb, bx, by, e1, y):
+    """(element offset, ok) of halo row y as mx::row_src computes them: k.off + row * BS0 with
+    row = s*y + d (clamped to 0 for a mirrored row past a narrow last block), and whether the
+    row lies inside its block's valid rows."""
     has_up, has_dn = by > 0, by + 1 < nb[1]
     du = bs[1] * (1 - nb[0]) if has_up else -1
     dd = bs[1] * (nb[0] - 1) if has_dn else 2 * e1 - 1
     up, dn = y < 0, y >= e1
     flip = (up and not has_up) or (dn and not has_dn)
     d = du if up else (dd if dn else 0)
+    row = (-y if flip else y) + d
+    if dn and not has_dn and row < 0:
+        row = 0
     koff = (by * nb[0] + bx) * bs[0] * bs[1]
-    return koff + ((-y if flip else y) + d) * bs[0]
+    if up:
+        ok = has_up or -y - 1 < e1
+    elif dn:
+        ok = (y - bs[1] < ext(ds, bs, by + 1, 1)) if has_dn else (2 * e1 - 1 - y >= 0)
+    else:
+        ok = True
+    return koff + row * bs[0], ok
 
 
 def tile_for(bs, tx, ty, vx):
@@ -76,7 +96,8 @@ def ldv_reads(ds, bs, nb, bx, by, x, y, vx, h0):
     i1 = 0 if y < 0 else (2 if y >= e1 else 1)
     if i0a == i0b:
         nbx, nby = bx + i0a - 1, by + i1 - 1
-        if 0 <= nbx < nb[0] and 0 <= nby < nb[1]:
+        if (0 <= nbx < nb[0] and 0 <= nby < nb[1] and modulo(x, bs[0]) + vx <= ext(ds, bs, nbx, 0)
+                and modulo(y, bs[1]) < ext(ds, bs, nby, 1)):
             xx, yy = modulo(x, bs[0]), modulo(y, bs[1])
             assert xx % vx == 0, "misaligned vector"
             return [(nby * nb[0] + nbx, xx + v, yy, True) for v in range(vx)]
@@ -99,7 +120,15 @@ GEOMS = [((512, 384), (128, 128), 1, 1, (256, 16, 4)),
          ((98, 60), (49, 20), 2, 1, (256, 16, 4)),
          ((2048, 1040), (1024, 1024), 1, 1, (1024, 16, 16)),
          ((300, 220), (128, 64), 1, 1, (512, 16, 16)),
-         ((200, 150), (64, 32), 1, 1, (256, 16, 16))]
+         ((200, 150), (64, 32), 1, 1, (256, 16, 16)),
+         # edge blocks narrower than the halo (the rule applied again from the narrow block)
+         ((65, 33), (32, 16), 2, 2, (256, 16, 4)),
+         ((129, 66), (64, 32), 3, 3, (256, 16, 4)),
+         ((33, 17), (16, 16), 3, 2, (256, 16, 4)),
+         ((3, 2), (4, 4), 3, 3, (256, 16, 4)),
+         ((2, 1), (8, 8), 1, 1, (256, 16, 4)),
+         ((1025, 1026), (1024, 1024), 3, 3, (1024, 16, 8)),
+         ((258, 130), (128, 64), 2, 2, (512, 16, 16))]
 
 
 @pytest.mark.parametrize("ds,bs,h0,h1,tile", GEOMS)
@@ -117,7 +146,6 @@ def test_focal_reads_in_bounds(ds, bs, h0, h1, tile):
         bx, by = b % nb[0], b // nb[0]
         e0 = min(bs[0], ds[0] - bx * bs[0])
         e1 = min(bs[1], ds[1] - by * bs[1])
-        assert e0 >= h0 and e1 >= h1
         for t in range(tb[0] * tb[1]):
             x0, y0 = (t % tb[0]) * tx, (t // tb[0]) * ty
             if x0 >= e0 or y0 >= e1:
@@ -142,7 +170,7 @@ def test_focal_reads_in_bounds(ds, bs, h0, h1, tile):
                         x = x0 - pad + j * vx if j < pv else x0 + tx + (j - pv) * vx
                     y = min(y0 - h1 + r, e1 + h1 - 1)
                     xc = max(0, min(x, bs[0] - vx))
-                    off = row_src_offset(bs, nb, bx, by, e1, y) + xc
+                    off = row_src(ds, bs, nb, bx, by, e1, y)[0] + xc
                     assert off % vx == 0 and 0 <= off and off + vx <= total, (b, t, kind, e)
                     blk = off // (bs[0] * bs[1])
                     assert blk == (off + vx - 1) // (bs[0] * bs[1]), "vector crosses a block"
@@ -154,7 +182,7 @@ def test_focal_reads_in_bounds(ds, bs, h0, h1, tile):
                 y = y0 - h1 + r
                 if y >= e1 + h1 or x + vx <= -h0 or x >= e0 + h0:
                     continue  # zero
-                if 0 <= x and x + vx <= e0:
+                if 0 <= x and x + vx <= e0 and row_src(ds, bs, nb, bx, by, e1, y)[1]:
                     # the fetched word: the row's block (central, above/below) or mirrored row
                     if 0 <= y < e1:
                         blk, yy = b, y
@@ -170,6 +198,9 @@ def test_focal_reads_in_bounds(ds, bs, h0, h1, tile):
                     reads = [(blk, x + v, yy, False) for v in range(vx)]
                 else:
                     reads = ldv_reads(ds, bs, nb, bx, by, x, y, vx, h0)
+                for rb, rx, ry, _vec in reads:  # every cell read is a valid cell of its block
+                    assert 0 <= rx < ext(ds, bs, rb % nb[0], 0) and 0 <= ry < ext(ds, bs, rb // nb[0], 1), \
+                        (b, t, kind, r, j, rb, rx, ry)
             # LDS window reads of every thread stay inside the halo tile
             for tid in range(256):
                 ly, lx = tid // twx, (tid % twx) * vx
@@ -188,17 +219,20 @@ def test_focal_reads_in_bounds(ds, bs, h0, h1, tile):
 
 
 def test_border_rule_matches_oracle():
-    """The emulated rule and the oracle's C restatement agree on every lookup."""
+    """The emulated rule and the oracle's C restatement agree on every lookup, also past edge
+    blocks narrower than the halo, and every lookup ends inside a block's valid extent."""
     from oracle import ref as O
-    for ds, bs in [((70, 41), (32, 16)), ((64, 64), (16, 16)), ((9, 7), (4, 4))]:
+    for ds, bs, h in [((70, 41), (32, 16), 1), ((64, 64), (16, 16), 1), ((9, 7), (4, 4), 1), ((65, 33), (32, 16), 3),
+                      ((9, 7), (4, 4), 4), ((3, 2), (4, 4), 4), ((1, 1), (8, 8), 3), ((17, 18), (8, 8), 7)]:
         nb = ((ds[0] + bs[0] - 1) // bs[0], (ds[1] + bs[1] - 1) // bs[1])
         for b in range(nb[0] * nb[1]):
             bx, by = b % nb[0], b // nb[0]
             e0 = min(bs[0], ds[0] - bx * bs[0])
             e1 = min(bs[1], ds[1] - by * bs[1])
-            for x in range(-1, e0 + 1):
-                for y in range(-1, e1 + 1):
+            for x in range(-h, e0 + h):
+                for y in range(-h, e1 + h):
                     blk, sx, sy = ldf_index(ds, bs, nb, bx, by, x, y)
+                    assert 0 <= sx < ext(ds, bs, blk % nb[0], 0) and 0 <= sy < ext(ds, bs, blk // nb[0], 1)
                     gx = (blk % nb[0]) * bs[0] + sx
                     gy = (blk // nb[0]) * bs[1] + sy
                     assert (gx, gy) == O.src_cell(ds, bs, bx, by, x, y)
@@ -297,7 +331,11 @@ def span_pad(h0, size):
                                                    ((300, 220), (128, 64), 1, 1, (128, 32, 4), 4),
                                                    ((70, 41), (32, 16), 2, 2, (128, 32, 4), 4),
                                                    ((101, 57), (52, 27), 2, 2, (256, 16, 4), 4),
-                                                   ((98, 60), (48, 20), 1, 1, (256, 32, 8), 1)])
+                                                   ((98, 60), (48, 20), 1, 1, (256, 32, 8), 1),
+                                                   ((65, 33), (32, 16), 2, 2, (128, 32, 4), 4),
+                                                   ((130, 67), (64, 32), 3, 3, (512, 16, 8), 1),
+                                                   ((3, 2), (8, 8), 3, 3, (128, 32, 4), 4),
+                                                   ((1025, 1026), (1024, 1024), 3, 3, (1024, 16, 8), 1)])
 def test_stream_reads_in_bounds(ds, bs, h0, h1, tile, size):
     """Emulates the global reads of stream_tile: every row a thread fetches
     (central rows directly, others through the row source or zeros) and its
@@ -331,20 +369,21 @@ def test_stream_reads_in_bounds(ds, bs, h0, h1, tile, size):
                 for y in rows:
                     if y >= e1 + h1:
                         continue  # zeros
-                    if 0 <= y < e1:
-                        blk, yy = b, y
-                    elif 0 <= by + (-1 if y < 0 else 1) < nb[1]:
-                        blk, yy = (by + (-1 if y < 0 else 1)) * nb[0] + bx, modulo(y, bs[1])
-                    else:
-                        blk, yy = b, invert(y, e1)
-                    be1 = min(bs[1], ds[1] - (blk // nb[0]) * bs[1])
-                    assert 0 <= yy < be1
-                    if x - a >= 0 and x + vx + a <= e0:
+                    off, ok = (((by * nb[0] + bx) * bs[1] + y) * bs[0], True) if 0 <= y < e1 else \
+                        row_src(ds, bs, nb, bx, by, e1, y)
+                    assert 0 <= off and off + bs[0] <= total  # the row source stays in the allocation
+                    blk, yy = off // (bs[0] * bs[1]), (off // bs[0]) % bs[1]
+                    if ok:
+                        be1 = min(bs[1], ds[1] - (blk // nb[0]) * bs[1])
+                        assert 0 <= yy < be1
+                    if x - a >= 0 and x + vx + a <= e0 and ok:
                         cells = [(blk, c, yy) for c in range(x - a, x + vx + a)]
+                        for c in range(x - a, x + vx + a):
+                            assert ldf_index(ds, bs, nb, bx, by, c, y) == (blk, c, yy)
                     else:
                         cells = []
                         for xc in range(x - a, x + vx + a):
-                            if 0 <= xc < e0:
+                            if 0 <= xc < e0 and ok:
                                 cells.append((blk, xc, yy))
                             elif -h0 <= xc < e0 + h0:
                                 cells.append(ldf_index(ds, bs, nb, bx, by, xc, y))
