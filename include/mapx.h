@@ -213,6 +213,14 @@ int ma_statsRange(ma_node_t *node, ma_variant_t *max, ma_variant_t *min);
 /* Synthetic DEM generated in HBM: ng Gaussians given as rows (cx, cy, sigma,
  * amplitude) plus value-noise fBm with the given seed; F32, block-tiled. */
 ma_node_t *ma_genDem(ma_dim2_t ds, ma_dim2_t bs, const double *gauss, int ng, unsigned long long noise_seed);
+/* ma_genDem into caller memory, allocating nothing (no counterpart in the reference; for callers
+ * that cannot allocate device memory: code under graph capture, serving loops, harnesses that
+ * forbid hipMalloc). `out` is device memory of at least out_bytes >= the block-tiled raster
+ * (ceil(ds/bs) blocks of bs cells, 4 bytes each); at most 256 Gaussians, whose table goes through
+ * a 10240-byte device array of the library that each GPU holds until the process ends. Calls
+ * that overlap on one GPU race on that array. Returns once the raster is written; 0 on success. */
+int ma_genDemNoAlloc(void *out, size_t out_bytes, ma_dim2_t ds, ma_dim2_t bs, const double *gauss, int ng,
+                     unsigned long long noise_seed);
 /* Voronoi zone raster of np integer points (px, py), type U8/U16/S32/U32. */
 ma_node_t *ma_genZones(ma_dim2_t ds, ma_dim2_t bs, ma_datatype_t dt, const int *px, const int *py, int np);
 /* Device time of the last evaluation in milliseconds (HIP events). */

@@ -16,6 +16,8 @@ namespace mapx { struct Node; }
 namespace mapx {
 namespace gen {
 size_t dem_scratch_bytes(int ng);
+size_t nalloc_scratch_bytes();
+void* nalloc_scratch(int device);
 size_t voronoi_scratch_bytes(int np);
 void dem(void* out, int BS0, int BS1, int NB0, int NB1, int DS0, int DS1, const double* params, int ng,
          uint64_t seed, void* scratch, void* stream);
@@ -559,6 +561,28 @@ ma_node_t* ma_genDem(ma_dim2_t ds, ma_dim2_t bs, const double* gauss, int ng, un
     r.adopt(n, h->dev);
     return n;
   }, (Node*)nullptr);
+}
+
+int ma_genDemNoAlloc(void* out, size_t out_bytes, ma_dim2_t ds, ma_dim2_t bs, const double* gauss, int ng,
+                     unsigned long long noise_seed) {
+  return guard([&] {
+    if (!dev::initialized()) throw std::runtime_error("genDemNoAlloc: no device (call ma_setupDevices)");
+    Meta m;
+    m.dt = F32; m.nd = D2; m.mo = ROW + BLK; m.ds = D(ds); m.bs = D(bs);
+    if (m.ds.arr[0] <= 0 || m.ds.arr[1] <= 0 || m.bs.arr[0] <= 0 || m.bs.arr[1] <= 0)
+      throw std::runtime_error("genDemNoAlloc: bad raster geometry");
+    const size_t max_ng = gen::nalloc_scratch_bytes() / gen::dem_scratch_bytes(1);
+    if (ng < 0 || (size_t)ng > max_ng)
+      throw std::runtime_error("genDemNoAlloc: at most " + std::to_string(max_ng) + " Gaussians (its scratch holds " +
+                               std::to_string(gen::nalloc_scratch_bytes()) + " bytes)");
+    if (ng > 0 && !gauss) throw std::runtime_error("genDemNoAlloc: null Gaussian table");
+    const size_t need = (size_t)m.nb(0) * m.nb(1) * m.bs.arr[0] * m.bs.arr[1] * sizeof(float);
+    if (!out || out_bytes < need)
+      throw std::runtime_error("genDemNoAlloc: the output needs " + std::to_string(need) + " bytes of device memory");
+    gen::dem(out, m.bs.arr[0], m.bs.arr[1], m.nb(0), m.nb(1), m.ds.arr[0], m.ds.arr[1], gauss, ng, noise_seed,
+             gen::nalloc_scratch(dev::device_id()), dev::stream());
+    return 0;
+  }, -1);
 }
 
 ma_node_t* ma_genZones(ma_dim2_t ds, ma_dim2_t bs, ma_datatype_t dt, const int* px, const int* py, int np) {

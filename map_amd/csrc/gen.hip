@@ -8,6 +8,7 @@
 // documented in one place; the oracle holds an independent CPU restatement.
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -163,6 +164,34 @@ void voronoi(void* out, int dt_bytes, bool is_signed, int BS0, int BS1, int NB0,
   const hipError_t le = hipGetLastError();
   check(hipStreamSynchronize(s), "sync");
   check(le, "k_voronoi launch");
+}
+
+// ---- scratch of the allocation-free DEM entry point --------------------------------
+// ma_genDemNoAlloc keeps the Gaussians' table (at most 256 rows) in this
+// file-scope device array instead of memory the library allocates. Each GPU
+// holds its own copy until the process ends; every call on a GPU overwrites
+// it, so calls that overlap on one GPU race. Host code never passes the
+// array's name to a kernel: its device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+static __device__ __align__(256) unsigned char scratch[256 * sizeof(Gauss)];
+static std::mutex g_scratch_mtx;
+static void* g_scratch[64] = {};
+
+size_t nalloc_scratch_bytes() { return sizeof(scratch); }
+
+void nalloc_scratch_init(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mtx);
+  if (device < 0 || device >= 64) throw std::runtime_error("gen: device index out of range for the scratch table");
+  if (g_scratch[device]) return;
+  void* p = nullptr;
+  check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+  g_scratch[device] = p;
+}
+
+void* nalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mtx);
+  if (device < 0 || device >= 64 || !g_scratch[device]) throw std::runtime_error("gen: scratch not set up on this device");
+  return g_scratch[device];
 }
 
 }  // namespace gen

@@ -17,6 +17,9 @@ namespace radial {  // radial.hip
 int run(void* out, const void* in, int dt_bytes, int op, int BS0, int BS1, int NB0, int NB1, int DS0, int DS1, int s0,
         int s1, void* stream);
 }  // namespace radial
+namespace gen {  // gen.hip
+void nalloc_scratch_init(int device);
+}  // namespace gen
 namespace spread {  // spread.hip
 int run(void* out, const void* in, const void* dir, void* cnt, int dt, int BS0, int BS1, int NB0, int NB1, int DS0,
         int DS1, void* stream);
@@ -378,6 +381,7 @@ void Runtime::setupDevices(const std::string& plat, unsigned dev_type, const std
   int device = 0;
   if (const char* e = std::getenv("MAPX_DEVICE")) device = std::atoi(e);
   dev::init(device);
+  gen::nalloc_scratch_init(device);  // the device address of the allocation-free generator's scratch
   read_env(opt);
   if (const char* p = std::getenv("MAPX_TUNE_CACHE")) tune_load(p);
 }
