@@ -385,6 +385,9 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   // one cell's outputs
   s << "  struct Out {\n";
   for (Node* o : d2_outs) s << "    " << CT(o->meta.dt) << " o" << LN(o) << ";\n";
+  for (size_t j = 0; j < ks.zouts.size(); j++)
+    if (ks.zouts[j].node->kind == Kind::ZonalStats)
+      s << "    " << CT(ks.zouts[j].dt) << " zv" << j << ";\n    int zz" << j << ";\n";
   s << "    int unused;\n  };\n";
   // accumulators
   s << "  struct Acc {\n";
@@ -631,11 +634,20 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     Node* src = z.node->prev[0];
     if (z.node->kind == Kind::ZonalReduc)
       s << "    if (ok) acc.r" << j << " = " << red_op(z.red) << "::f(acc.r" << j << ", (" << CT(z.acc_dt) << ")" << V(src) << ");\n";
-    else
-      s << "    mx::zone_step<" << CT(z.dt) << ", " << z.nzones << ">(acc.z" << j << ", acc.q" << j << ", ok, " << V(src)
-        << ", (int)" << V(z.node->prev[1]) << ");\n";
+    else  // accumulated for the thread's VX cells of the row at once (zone_row)
+      s << "    out.zv" << j << " = (" << CT(z.dt) << ")" << V(src) << ";\n    out.zz" << j << " = (int)" << V(z.node->prev[1]) << ";\n";
   }
   s << "    (void)a; (void)k; (void)x; (void)y; (void)ok; (void)W; (void)C; (void)v; (void)acc; (void)out;\n  }\n";
+  // zone_row(): the per-zone statistics of the thread's VX cells of one row, in cell order
+  s << "  template <int VX> __device__ static __forceinline__ void zone_row(Acc& acc, const Out (&o)[VX], unsigned okm) {\n";
+  for (size_t j = 0; j < ks.zouts.size(); j++) {
+    const ZonalOut& z = ks.zouts[j];
+    if (z.node->kind != Kind::ZonalStats) continue;
+    s << "    {\n      " << CT(z.dt) << " zv[VX];\n      int zz[VX];\n#pragma unroll\n      for (int v = 0; v < VX; v++) { zv[v] = o[v].zv"
+      << j << "; zz[v] = o[v].zz" << j << "; }\n      mx::zone_steps<" << CT(z.dt) << ", " << z.nzones << ", VX>(acc.z" << j
+      << ", acc.q" << j << ", okm, zv, zz);\n    }\n";
+  }
+  s << "    (void)acc; (void)o; (void)okm;\n  }\n";
   // flush(): writes one chunk's partial record and restarts the accumulators
   s << "  static constexpr bool ZONAL = " << (zonal.empty() ? "false" : "true") << ";\n";
   s << "  static constexpr unsigned PART_BYTES = " << part_bytes << "u;\n";
