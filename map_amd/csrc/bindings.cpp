@@ -342,7 +342,11 @@ int ma_setOption(const char* key, long long value) {
     else if (k == "tile_st") o.tile.ST = value != 0;
     else if (k == "libm") o.libm = value != 0;
     else if (k == "predict") o.predict = value != 0;
-    else if (k == "wg_per_cu") {
+    else if (k == "hbm_cap") {
+      if (value < 0) throw std::runtime_error("setOption: hbm_cap is a byte count (0: no cap)");
+      o.hbm_cap = (size_t)value;
+      R().store.cap = o.hbm_cap;
+    } else if (k == "wg_per_cu") {
       if (value < 0 || value > 64) throw std::runtime_error("setOption: wg_per_cu in [0, 64]");
       o.wg_per_cu = (int)value;
     } else if (k == "batch_tasks") o.batch_tasks = value != 0;
@@ -373,6 +377,7 @@ long long ma_getOption(const char* key) {
   if (k == "tile_st") return o.tile.ST;
   if (k == "libm") return o.libm;
   if (k == "predict") return o.predict;
+  if (k == "hbm_cap") return (long long)o.hbm_cap;
   if (k == "wg_per_cu") return o.wg_per_cu;
   if (k == "batch_tasks") return o.batch_tasks;
   if (k == "queue") return o.queue;
@@ -420,13 +425,16 @@ const char* ma_runStats(void) {
   std::ostringstream s;
   s << "{\"evals\": " << r.cnt.evals << ", \"groups\": " << r.cnt.groups << ", \"launches\": " << r.cnt.launches
     << ", \"compiles\": " << r.cnt.compiles << ", \"shape_hits\": " << r.cnt.shape_hits
-    << ", \"predicted\": " << r.cnt.predicted
+    << ", \"predicted\": " << r.cnt.predicted << ", \"spills\": " << r.cnt.spills
+    << ", \"unspills\": " << r.cnt.unspills << ", \"dropped_for_space\": " << r.cnt.dropped_for_space
+    << ", \"bytes_spilled\": " << r.cnt.bytes_spilled
     << ", \"scalar_tasks\": " << r.cnt.scalar_tasks << ", \"bytes_h2d\": " << r.cnt.bytes_h2d
     << ", \"bytes_d2h\": " << r.cnt.bytes_d2h << ", \"store_in_use\": " << r.store.in_use()
     << ", \"store_cached\": " << r.store.cached() << ", \"graphs\": " << dev::graph_cache_size()
     << ", \"ms_last_eval\": " << r.cnt.ms_last_eval << ", \"ms_fuse\": " << r.cnt.ms_fuse
     << ", \"ms_codegen\": " << r.cnt.ms_codegen << ", \"ms_launch\": " << r.cnt.ms_launch
-    << ", \"ms_plan\": " << r.cnt.ms_plan << ", \"ms_prepare\": " << r.cnt.ms_prepare << ", \"occupancy\": {";
+    << ", \"ms_plan\": " << r.cnt.ms_plan << ", \"ms_prepare\": " << r.cnt.ms_prepare
+    << ", \"ms_collect\": " << r.cnt.ms_collect << ", \"ms_predict\": " << r.cnt.ms_predict << ", \"occupancy\": {";
   bool first = true;
   for (auto& k : r.kernel_occ_) { s << (first ? "" : ", ") << "\"" << k.first << "\": " << k.second; first = false; }
   s << "}, \"cus\": " << (dev::initialized() ? dev::num_cus() : 0) << "}";

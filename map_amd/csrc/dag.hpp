@@ -72,7 +72,8 @@ struct Node {
   // Set during one evaluation for a node whose value is already resident in
   // HBM (D2) or known (D0): it enters the evaluation as an input.
   bool cut = false;
-
+  unsigned vstamp = 0, tstamp = 0;  // per-listing visit / target marks of the runtime
+  unsigned long long use = 0;        // the runtime's tick of the last evaluation that listed the node
   int pattern() const;
   bool isInput() const { return kind == Kind::Read || cut; }
   bool isOutput() const { return kind == Kind::Write || kind == Kind::Scalar; }

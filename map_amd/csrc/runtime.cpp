@@ -43,18 +43,35 @@ struct HostTimer {
 
 void* Store::alloc(size_t bytes) {
   bytes = (bytes + 255) & ~size_t(255);
+  if (cap) {
+    // under the cap the buffers in use and this one must fit: resident rasters move out first
+    while (in_use_ + bytes > cap)
+      if (bytes > cap || !reclaim || !reclaim())
+        fail("store: " + std::to_string(bytes) + " bytes do not fit the HBM cap of " + std::to_string(cap) +
+             " bytes (" + std::to_string(in_use_) + " in use by the evaluation)");
+  }
   auto it = free_.find(bytes);
-  void* p;
+  void* p = nullptr;
   if (it != free_.end()) {
     p = it->second;
     free_.erase(it);
     cached_ -= bytes;
-  } else {
+    size_[p] = bytes;
+    in_use_ += bytes;
+    if (cap && in_use_ + cached_ > cap) trim();  // the other cached buffers beyond the cap go back
+    return p;
+  }
+  // no room on the device: cached buffers go back first, then resident rasters
+  // move out (each round frees memory or ends in the error)
+  for (;;) {
+    if (cap && in_use_ + cached_ + bytes > cap) trim();  // leaves in_use_ + bytes <= cap
     try {
       p = dev::malloc_device(bytes);
+      break;
     } catch (const std::exception&) {
-      trim();  // give cached buffers back and retry once
-      p = dev::malloc_device(bytes);
+      if (!free_.empty()) { trim(); continue; }
+      if (reclaim && reclaim()) continue;
+      throw;
     }
   }
   size_[p] = bytes;
@@ -67,6 +84,15 @@ void Store::release(void* p) {
   if (it == size_.end()) fail("store: release of an unknown buffer");
   free_.emplace(it->second, p);
   cached_ += it->second;
+  in_use_ -= it->second;
+  size_.erase(it);
+}
+
+// The caller has made sure no enqueued or recorded work reads p any more.
+void Store::discard(void* p) {
+  auto it = size_.find(p);
+  if (it == size_.end()) fail("store: discard of an unknown buffer");
+  dev::free_device(p);
   in_use_ -= it->second;
   size_.erase(it);
 }
@@ -372,6 +398,7 @@ static void read_env(Options& o) {
   }
   if (const char* e = std::getenv("MAPX_GRAPH")) o.graph = std::atoi(e) != 0;
   if (const char* e = std::getenv("MAPX_FUSE_FOCAL_ZONAL")) o.focal_zonal = std::atoi(e) != 0;
+  if (const char* e = std::getenv("MAPX_HBM_CAP")) o.hbm_cap = (size_t)std::strtoull(e, nullptr, 10);
 }
 
 void Runtime::setupDevices(const std::string& plat, unsigned dev_type, const std::string& dev_name) {
@@ -383,6 +410,8 @@ void Runtime::setupDevices(const std::string& plat, unsigned dev_type, const std
   dev::init(device);
   gen::nalloc_scratch_init(device);  // the device address of the allocation-free generator's scratch
   read_env(opt);
+  store.cap = opt.hbm_cap;
+  store.reclaim = [this] { return spill_one(); };
   if (const char* p = std::getenv("MAPX_TUNE_CACHE")) tune_load(p);
 }
 
@@ -392,11 +421,26 @@ size_t Runtime::rasterBytes(const Meta& m) const {
 
 void* Runtime::residentPtr(Node* n) {
   auto it = resident_.find(n);
-  if (it == resident_.end()) fail("raster of node " + std::to_string(n->id) + " is not resident");
+  if (it == resident_.end()) {
+    if (!spilled_.count(n)) fail("raster of node " + std::to_string(n->id) + " is not resident");
+    unspill(n);
+    it = resident_.find(n);
+  }
   return it->second;
 }
 
+void Runtime::forget_spilled(Node* n) {
+  auto sp = spilled_.find(n);
+  if (sp == spilled_.end()) return;
+  dev::free_host(sp->second.host);
+  spilled_.erase(sp);
+}
+
 void Runtime::drop(Node* n) {
+  if (spilled_.count(n)) {
+    if (!spilled_[n].generated || n->ref <= 0) forget_spilled(n);
+    return;
+  }
   auto it = resident_.find(n);
   if (it == resident_.end()) return;
   if (pins_.count(n)) return;  // an admitted request still reads it
@@ -409,6 +453,7 @@ void Runtime::drop(Node* n) {
 static int ext_refs(const Node* n) { return n->ref - (int)n->next.size(); }
 
 void Runtime::forget(Node* n) {
+  forget_spilled(n);
   auto it = resident_.find(n);
   if (it == resident_.end()) return;
   store.release(it->second);
@@ -486,7 +531,84 @@ std::string Runtime::cacheStats() const {
 void Runtime::release_unreferenced() {
   std::vector<Node*> gone;
   for (auto& r : resident_) if (r.first->ref <= 0) gone.push_back(r.first);
+  for (auto& r : spilled_) if (r.first->ref <= 0) gone.push_back(r.first);
   for (Node* n : gone) drop(n);
+}
+
+// ---- spilling: resident rasters out of HBM and back ---------------------------------
+
+// Called by the store when an allocation finds no room. The victim is the
+// raster whose last evaluation is the oldest among those that no evaluation or
+// admitted request in flight lists (Runtime::LiveTick) and no request pins. A
+// raster that can be made again from its source (a file, a host array) or that
+// nothing references any more leaves HBM; any other (a computed or generated
+// raster still referenced) is copied to pinned host memory first. Enqueued and
+// recorded work is drained before the buffer goes back to the device.
+bool Runtime::spill_one() {
+  const unsigned long long floor = live_ticks_.empty() ? ~0ull : *live_ticks_.begin();
+  Node* v = nullptr;
+  for (auto& r : resident_) {
+    Node* n = r.first;
+    if (n->use >= floor || pins_.count(n)) continue;
+    if (!v || n->use < v->use || (n->use == v->use && n->id < v->id)) v = n;
+  }
+  if (!v) return false;
+  void* d = resident_[v];
+  const size_t bytes = rasterBytes(v->meta);
+  const bool generated = v->host && v->host->dev == d;
+  const bool remade = !generated && v->kind == Kind::Read;  // a file or a host array
+  PauseRecording pause;
+  if (remade || v->ref <= 0) {
+    dev::sync();
+    cnt.dropped_for_space++;
+  } else {
+    void* h = dev::malloc_host(bytes);
+    try {
+      dev::d2h(h, d, bytes);
+      dev::sync();
+    } catch (...) {
+      dev::free_host(h);
+      throw;
+    }
+    spilled_[v] = Spilled{h, generated};
+    if (generated) v->host->dev = nullptr;
+    cnt.spills++;
+    cnt.bytes_spilled += (long)bytes;
+  }
+  resident_.erase(v);
+  store.discard(d);
+  return true;
+}
+
+unsigned long long Runtime::tick_begin(const std::vector<Node*>& list) {
+  const unsigned long long t = ++use_clock_;
+  live_ticks_.insert(t);
+  for (Node* n : list) n->use = t;
+  return t;
+}
+
+void Runtime::tick_end(unsigned long long t) {
+  auto it = live_ticks_.find(t);
+  if (it != live_ticks_.end()) live_ticks_.erase(it);
+}
+
+void Runtime::unspill(Node* n) {
+  auto sp = spilled_.find(n);
+  if (sp == spilled_.end()) fail("unspill: raster of node " + std::to_string(n->id) + " is not spilled");
+  const size_t bytes = rasterBytes(n->meta);
+  void* d = store.alloc(bytes);  // may move other rasters out; this one is not resident
+  sp = spilled_.find(n);
+  {
+    PauseRecording pause;
+    dev::h2d(d, sp->second.host, bytes);
+    dev::sync();
+  }
+  if (sp->second.generated) n->host->dev = d;
+  dev::free_host(sp->second.host);
+  spilled_.erase(sp);
+  resident_[n] = d;
+  cnt.unspills++;
+  cnt.bytes_h2d += (long)bytes;
 }
 
 // Staging chunk: files and host rasters move through two pinned buffers of
@@ -507,6 +629,7 @@ void Runtime::staged(int i) { stage_ev_[i] = dev::event_record(); }
 
 void Runtime::ensure_input(Node* n) {
   if (resident_.count(n)) return;
+  if (spilled_.count(n)) { unspill(n); return; }
   if (n->kind != Kind::Read) fail("input raster of node " + std::to_string(n->id) + " is not available");
   const size_t bytes = rasterBytes(n->meta);
   if (n->host && n->host->dev) fail("generated raster of node " + std::to_string(n->id) + " was released");
@@ -541,6 +664,11 @@ void Runtime::ensure_input(Node* n) {
 void Runtime::fetch(Node* n, void* host, size_t bytes) {
   std::lock_guard<std::recursive_mutex> lk(mtx_);
   if (bytes != rasterBytes(n->meta)) fail("fetch: byte count differs from the raster size");
+  auto sp = spilled_.find(n);
+  if (sp != spilled_.end()) {  // moved out of HBM: the host copy is the raster
+    std::memcpy(host, sp->second.host, bytes);
+    return;
+  }
   void* d = residentPtr(n);
   PauseRecording pause;
   dev::d2h(host, d, bytes);
@@ -574,14 +702,16 @@ void Runtime::write_file(Node* w) {
 }
 
 std::vector<Node*> Runtime::cut_list(const std::vector<Node*>& targets) {
-  std::set<Node*> seen, tset(targets.begin(), targets.end());
+  const unsigned st = ++visit_stamp_;
+  for (Node* t : targets) t->tstamp = st;  // targets are listed, never cut
   std::vector<Node*> stack(targets.begin(), targets.end()), out;
   while (!stack.empty()) {
     Node* n = stack.back(); stack.pop_back();
-    if (!seen.insert(n).second) continue;
+    if (n->vstamp == st) continue;
+    n->vstamp = st;
     out.push_back(n);
-    if (!tset.count(n)) {
-      bool have = (n->meta.nd == D2 && resident_.count(n)) ||
+    if (n->tstamp != st) {
+      bool have = (n->meta.nd == D2 && (resident_.count(n) || spilled_.count(n))) ||
                   (n->meta.nd == D0 && n->evaluated && n->kind != Kind::Constant);
       if (have && n->kind != Kind::Read) { n->cut = true; continue; }
     }
@@ -594,17 +724,19 @@ std::vector<Node*> Runtime::cut_list(const std::vector<Node*>& targets) {
   return out;
 }
 
+// `targets` sorted by address (Eval::tsorted): a group's nodes are looked up
+// in it, so the cost does not grow with the number of targets.
 std::vector<Node*> Runtime::d2_outputs(const Group& g, const std::vector<Node*>& targets) {
-  std::set<Node*> s;
+  std::vector<Node*> v;
   for (Node* n : g.outs) {
-    if (n->kind == Kind::Write) s.insert(n->prev[0]);
+    if (n->kind == Kind::Write) v.push_back(n->prev[0]);
     else if (n->kind == Kind::Scalar) continue;
-    else if (n->meta.nd == D2) s.insert(n);
+    else if (n->meta.nd == D2) v.push_back(n);
   }
-  for (Node* t : targets)
-    if (t->meta.nd == D2 && std::find(g.nodes.begin(), g.nodes.end(), t) != g.nodes.end()) s.insert(t);
-  std::vector<Node*> v(s.begin(), s.end());
+  for (Node* t : g.nodes)
+    if (t->meta.nd == D2 && std::binary_search(targets.begin(), targets.end(), t)) v.push_back(t);
   std::sort(v.begin(), v.end(), [](Node* a, Node* b) { return a->id < b->id; });
+  v.erase(std::unique(v.begin(), v.end()), v.end());
   return v;
 }
 
@@ -742,7 +874,8 @@ struct Runtime::Task {
 std::vector<Node*> Runtime::kernel_outputs(const Group& g, const std::vector<Node*>& targets, bool& has_zonal) {
   std::vector<Node*> outs = d2_outputs(g, targets), kouts;
   for (Node* n : outs)
-    if (std::find(g.nodes.begin(), g.nodes.end(), n) != g.nodes.end() && !resident_.count(n)) kouts.push_back(n);
+    if (std::find(g.nodes.begin(), g.nodes.end(), n) != g.nodes.end() && !resident_.count(n) && !spilled_.count(n))
+      kouts.push_back(n);
   has_zonal = false;
   for (Node* n : g.nodes) if (n->kind == Kind::ZonalReduc || n->kind == Kind::ZonalStats) has_zonal = true;
   return kouts;
@@ -760,10 +893,10 @@ Node* Runtime::radial_node(const Group& g) {
 void Runtime::radial_task(const Group& g, Eval& e) {
   Node* n = radial_node(g);
   if (g.nodes.size() != 1) fail("radial scan fused with other nodes");
+  bool zonal = false;
+  if (kernel_outputs(g, e.tsorted, zonal).empty()) return;
   for (Node* in : g.ins)
     if (in->meta.nd == D2) ensure_input(in);
-  bool zonal = false;
-  if (kernel_outputs(g, e.targets, zonal).empty()) return;
   const Meta& m = n->meta;
   void* out = store.alloc(rasterBytes(m));
   resident_[n] = out;
@@ -864,6 +997,10 @@ Variant window_fold(const Node* n, const Variant& v) {
 
 int Runtime::predict(const std::vector<Node*>& targets) {
   if (!opt.predict) return 0;
+  bool any = false;  // a zonal reduction still to compute: the only node predicted
+  for (Node* n : reach(targets))
+    if (n->kind == Kind::ZonalReduc && !n->evaluated) { any = true; break; }
+  if (!any) return 0;
   std::unordered_map<const Node*, Variant> cval;
   int done = 0;
   for (Node* n : list_upwards(targets)) {
@@ -932,10 +1069,30 @@ int Runtime::predict(const std::vector<Node*>& targets) {
   return done;
 }
 
+const std::vector<Node*>& Runtime::reach(const std::vector<Node*>& roots) {
+  const unsigned st = ++visit_stamp_;
+  reach_.clear();
+  reach_stack_.assign(roots.begin(), roots.end());
+  while (!reach_stack_.empty()) {
+    Node* n = reach_stack_.back();
+    reach_stack_.pop_back();
+    if (n->vstamp == st) continue;
+    n->vstamp = st;
+    reach_.push_back(n);
+    for (Node* p : n->prev) reach_stack_.push_back(p);
+  }
+  return reach_;
+}
+
 std::vector<Node*> Runtime::with_aux(const std::vector<Node*>& targets) {
+  std::vector<Node*> owners;
+  for (Node* n : reach(targets))
+    if (!n->aux.empty()) owners.push_back(n);
   std::vector<Node*> out = targets;
-  std::set<Node*> have(targets.begin(), targets.end());
-  for (Node* n : list_upwards(targets))
+  if (owners.empty()) return out;
+  std::sort(owners.begin(), owners.end(), [](Node* a, Node* b) { return a->id < b->id; });
+  std::unordered_set<Node*> have(targets.begin(), targets.end());
+  for (Node* n : owners)
     for (Node* a : n->aux)
       if (have.insert(a).second) out.push_back(a);
   return out;
@@ -963,10 +1120,10 @@ Node* Runtime::spread_node(const Group& g) {
 void Runtime::spread_task(const Group& g, Eval& e) {
   Node* n = spread_node(g);
   if (g.nodes.size() != 1) fail("spread scan fused with other nodes");
+  bool zonal = false;
+  if (kernel_outputs(g, e.tsorted, zonal).empty()) return;
   for (Node* in : g.ins)
     if (in->meta.nd == D2) ensure_input(in);
-  bool zonal = false;
-  if (kernel_outputs(g, e.targets, zonal).empty()) return;
   const Meta& m = n->meta;
   void* out = store.alloc(rasterBytes(m));
   resident_[n] = out;
@@ -988,19 +1145,24 @@ bool Runtime::prepare_task(const Group& g, Eval& e, Task& t) {
     kep = pre->second.second;
     has_zonal = kep && !kep->spec.zouts.empty();
   } else {
-    kouts = kernel_outputs(g, e.targets, has_zonal);
+    kouts = kernel_outputs(g, e.tsorted, has_zonal);
+  }
+  if (kouts.empty() && !has_zonal) {  // every output is held already: only inputs asked for are loaded
+    for (Node* n : g.ins)
+      if (n->meta.nd == D2 && n->kind == Kind::Read && e.tset.count(n)) ensure_input(n);
+    return false;
   }
   for (Node* n : g.ins) {
     if (n->meta.nd == D2) ensure_input(n);
     else if (!n->evaluated) fail("D0 input of node " + std::to_string(n->id) + " is not evaluated");
   }
-  if (kouts.empty() && !has_zonal) return false;
 
   const Meta gm = g.meta();
   for (Node* n : kouts) {
     if (!dim2_eq(n->meta.ds, gm.ds) || !dim2_eq(n->meta.bs, gm.bs))
       fail("group outputs differ in raster geometry");
-    resident_[n] = store.alloc(rasterBytes(n->meta));
+    void* d = store.alloc(rasterBytes(n->meta));  // may move idle rasters out of HBM
+    resident_[n] = d;
   }
   KernelEntry& ke = kep ? *kep : kernel_for(e.plan.get(), g, kouts);
   const KernelSpec& ks = ke.spec;
@@ -1036,14 +1198,17 @@ bool Runtime::prepare_task(const Group& g, Eval& e, Task& t) {
 }
 
 // Device memory outside the store (task tables, the chunk queue): when the
-// device is full, the store's cached buffers are returned and the allocation
-// is tried once more.
+// device is full, the store's cached buffers are returned, then resident
+// rasters move out of HBM, as for the store's own allocations.
 static void* alloc_outside_store(Store& st, size_t bytes) {
-  try {
-    return dev::malloc_device(bytes);
-  } catch (const std::exception&) {
-    st.trim();
-    return dev::malloc_device(bytes);
+  for (;;) {
+    try {
+      return dev::malloc_device(bytes);
+    } catch (const std::exception&) {
+      if (st.cached()) { st.trim(); continue; }
+      if (st.reclaim && st.reclaim()) continue;
+      throw;
+    }
   }
 }
 
@@ -1153,7 +1318,13 @@ void Runtime::fold_results(const std::vector<Task*>& batch, void* parts, void* t
   rb.host = (unsigned char*)pinned_alloc((size_t)T * PB);
   cnt.bytes_d2h += (long)((size_t)T * PB);
   if (async) {
-    dev::d2h(rb.host, res, (size_t)T * PB);
+    // the copy follows the fold launches, which are enqueued first when recorded
+    if (dev::recording()) {
+      PauseRecording pause;
+      dev::d2h(rb.host, res, (size_t)T * PB);
+    } else {
+      dev::d2h(rb.host, res, (size_t)T * PB);
+    }
     async->push_back(std::move(rb));
   } else {
     PauseRecording pause;
@@ -1266,35 +1437,45 @@ void Runtime::run_levels(const std::vector<std::pair<Eval*, size_t>>& work, std:
   size_t n = 0;
   for (auto& w : work) n += w.first->levels[w.second].size();
   std::vector<Task> tasks(n);
-  std::vector<Task*> order;
+  // the groups in order; unrecorded, a long list goes out in sub-batches of
+  // kSubBatch groups, each launched once prepared
+  std::vector<std::pair<Eval*, const Group*>> groups;
+  groups.reserve(n);
+  for (auto& w : work)
+    for (const Group* gp : w.first->levels[w.second]) groups.push_back({w.first, gp});
+  const size_t sub = dev::recording() || opt.form == FORM_PLAIN ? groups.size() : (size_t)kSubBatch;
   size_t i = 0;
-  {
-    HostTimer ht_prep(cnt.ms_prepare);
-    for (auto& w : work)
-      for (const Group* gp : w.first->levels[w.second]) {
-        const Group& g = *gp;
+  for (size_t g0 = 0; g0 < groups.size(); g0 += sub) {
+    const size_t g1 = std::min(groups.size(), g0 + sub);
+    std::vector<Task*> order;
+    {
+      HostTimer ht_prep(cnt.ms_prepare);
+      for (size_t k = g0; k < g1; k++) {
+        const Group& g = *groups[k].second;
+        Eval& e = *groups[k].first;
         cnt.groups++;
         Task& t = tasks[i++];
         if (host_group(g)) { scalar_task(g); continue; }
-        if (radial_node(g)) { radial_task(g, *w.first); continue; }
-        if (spread_node(g)) { spread_task(g, *w.first); continue; }
-        if (prepare_task(g, *w.first, t)) order.push_back(&t);
+        if (radial_node(g)) { radial_task(g, e); continue; }
+        if (spread_node(g)) { spread_task(g, e); continue; }
+        if (prepare_task(g, e, t)) order.push_back(&t);
       }
-  }
-  // tasks of one kernel form one batch (in order of first appearance)
-  std::vector<std::vector<Task*>> batches;
-  std::unordered_map<std::string, size_t> at;
-  for (Task* t : order) {
-    const bool one = opt.form == FORM_PLAIN || !opt.batch_tasks;
-    auto it = one ? at.end() : at.find(t->ke->spec.name);
-    if (it == at.end()) {
-      if (!one) at[t->ke->spec.name] = batches.size();
-      batches.push_back({t});
-    } else {
-      batches[it->second].push_back(t);
     }
+    // tasks of one kernel form one batch (in order of first appearance)
+    std::vector<std::vector<Task*>> batches;
+    std::unordered_map<std::string, size_t> at;
+    for (Task* t : order) {
+      const bool one = opt.form == FORM_PLAIN || !opt.batch_tasks;
+      auto it = one ? at.end() : at.find(t->ke->spec.name);
+      if (it == at.end()) {
+        if (!one) at[t->ke->spec.name] = batches.size();
+        batches.push_back({t});
+      } else {
+        batches[it->second].push_back(t);
+      }
+    }
+    for (auto& b : batches) run_batch(b, async);
   }
-  for (auto& b : batches) run_batch(b, async);
   for (auto& w : work)
     for (const Group* g : w.first->levels[w.second])
       for (Node* n : g->outs)
@@ -1384,6 +1565,8 @@ void Runtime::eval_plan(Eval& e) {
     for (Node* n : g->ins) if (n->meta.nd == D2) e.uses[n]++;
   e.tset.insert(e.targets.begin(), e.targets.end());
   for (Node* t : e.targets) if (t->kind == Kind::Write) e.tset.insert(t->prev[0]);
+  e.tsorted = e.targets;
+  std::sort(e.tsorted.begin(), e.tsorted.end());
   // levels: a group runs one level after the last group it depends on; the
   // groups of one level are independent, and those sharing a kernel are batched
   std::unordered_map<const Group*, int> lvl;
@@ -1424,20 +1607,30 @@ void Runtime::evaluate(const std::vector<Node*>& targets) {
   if (!dev::initialized()) fail("evaluate: devices not set up (call setupDevices first)");
   if (targets.empty()) fail("evaluate: nothing to evaluate");
   if (serving()) fail("evaluate: a server is open (submit requests to it, or close it first)");
-  release_unreferenced();
-  dag.collect_grown([this](Node* n) { forget(n); });
+  {
+    HostTimer ht(cnt.ms_collect);
+    release_unreferenced();
+    dag.collect_grown([this](Node* n) { forget(n); });
+  }
   cnt.evals++;
   dev::Timer t;
   t.start();
 
   Eval e;
-  e.targets = with_aux(targets);
-  predict(e.targets);
   {
+    HostTimer ht(cnt.ms_predict);
+    e.targets = with_aux(targets);
+    predict(e.targets);
+  }
+  {
+    // known values and rasters already held (in HBM or spilled) are not evaluated again
     std::vector<Node*> left;
-    for (Node* n : e.targets)
-      if (!(n->meta.nd == D0 && n->evaluated && n->kind != Kind::Constant)) left.push_back(n);
-    if (left.empty()) {  // every target is a known value
+    for (Node* n : e.targets) {
+      if (n->meta.nd == D0 && n->evaluated && n->kind != Kind::Constant) continue;
+      if (n->meta.nd == D2 && n->kind != Kind::Read && (resident_.count(n) || spilled_.count(n))) continue;
+      left.push_back(n);
+    }
+    if (left.empty()) {  // every target is a known value or a held raster
       t.stop();
       cnt.ms_last_eval = t.ms();
       return;
@@ -1449,16 +1642,48 @@ void Runtime::evaluate(const std::vector<Node*>& targets) {
     ~Uncut() { r->eval_uncut(e); }
   } uncut{this, e};
   eval_plan(e);
+  // the evaluation's rasters stay in HBM while it runs (spill_one passes them over)
+  struct Live {
+    Runtime* r; unsigned long long t;
+    ~Live() { r->tick_end(t); }
+  } live{this, tick_begin(e.list)};
 
+  // zonal results come back as pending copies, handed out before a level that
+  // reads one of them and at the end (the host does not wait for each fold)
+  std::vector<Readback> pend;
+  auto complete = [&] {
+    if (pend.empty()) return;
+    PauseRecording pause;
+    dev::sync();
+    for (Readback& rb : pend) distribute(rb);
+    pend.clear();
+  };
   if (opt.graph) dev::record_begin();
   try {
     for (size_t l = 0; l < e.levels.size(); l++) {
-      run_levels({{&e, l}}, nullptr);
+      bool reads_pending = false;
+      for (const Group* g : e.levels[l])
+        for (Node* n : g->ins)
+          if (n->meta.nd == D0 && !n->evaluated) reads_pending = true;
+      if (reads_pending) complete();
+      // a level of many tasks runs unrecorded, in sub-batches launched as they
+      // are prepared (the device starts while the host prepares the rest);
+      // chains of small levels (the generations of Life) accumulate into one graph
+      const bool big = e.levels[l].size() >= (size_t)kBigLevel;
+      const bool rec = dev::recording();
+      if (big && rec) dev::record_flush();
+      run_levels({{&e, l}}, &pend);
+      if (big && rec) dev::record_begin();
       eval_release(e, l);
     }
     if (dev::recording()) dev::record_flush();
+    complete();
   } catch (...) {
     if (dev::recording()) { try { dev::record_flush(); } catch (...) {} }
+    for (Readback& rb : pend) pinned_release(rb.host);  // copies into them are ordered before any reuse
+    // rasters this evaluation was computing may be partial: none is kept
+    for (Node* n : e.list)
+      if (!n->cut && n->kind != Kind::Read && n->meta.nd == D2) forget(n);
     throw;
   }
   eval_finish(e);
@@ -1469,6 +1694,8 @@ void Runtime::evaluate(const std::vector<Node*>& targets) {
 std::string Runtime::plan(const std::vector<Node*>& targets_in) {
   std::lock_guard<std::recursive_mutex> lk(mtx_);
   const std::vector<Node*> targets = with_aux(targets_in);  // as an evaluation would run them
+  std::vector<Node*> tsorted = targets;
+  std::sort(tsorted.begin(), tsorted.end());
   std::vector<Node*> list = cut_list(targets);
   struct Uncut {
     std::vector<Node*>& l;
@@ -1520,7 +1747,7 @@ std::string Runtime::plan(const std::vector<Node*>& targets_in) {
       continue;
     }
     if (!host_group(*g)) {
-      std::vector<Node*> outs = d2_outputs(*g, targets), kouts;
+      std::vector<Node*> outs = d2_outputs(*g, tsorted), kouts;
       for (Node* n : outs)
         if (std::find(g->nodes.begin(), g->nodes.end(), n) != g->nodes.end()) kouts.push_back(n);
       bool has_zonal = false;

@@ -9,6 +9,7 @@
 #pragma once
 #include "codegen.hpp"
 #include "dag.hpp"
+#include <functional>
 #include <list>
 #include <map>
 #include <memory>
@@ -16,6 +17,7 @@
 #include <set>
 #include <string>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 namespace mapx {
@@ -37,6 +39,7 @@ struct Options {
   TileShape tile{0, 0, 0, 0, 0}; // forced tile shape (TX 0 = per-group default / tuned)
   bool libm = false;          // FP32 sqrt / atan2 / sin-cos pairs through the device library, not the fast paths
   bool predict = true;        // zonal reductions of constant rasters computed on the host (no kernel)
+  size_t hbm_cap = 0;         // bytes of HBM the raster store may hold (0: as much as the device gives)
 };
 
 struct Counters {
@@ -44,21 +47,30 @@ struct Counters {
   long shape_hits = 0;  // kernels taken from the shape cache instead of generated
   long predicted = 0;   // zonal reductions of constant rasters known without a kernel
   long bytes_h2d = 0, bytes_d2h = 0;
+  long spills = 0, unspills = 0, dropped_for_space = 0;  // resident rasters moved out of HBM and back
+  long bytes_spilled = 0;
   double ms_last_eval = 0;
   // host time per phase (the reference's Clock regions, runtime/Clock.hpp:24-27)
   double ms_fuse = 0, ms_codegen = 0, ms_launch = 0, ms_eval_host = 0;
   double ms_plan = 0, ms_prepare = 0;  // evaluation planning (cut, plan lookup, levels); task preparation
+  double ms_collect = 0, ms_predict = 0;  // releases and DAG collection before an evaluation; aux targets + prediction
 };
 
 // Stream-ordered caching allocator: all device work runs on one in-order
-// stream, so a released buffer can be handed to later work at once.
+// stream, so a released buffer can be handed to later work at once. When the
+// device (or the cap) has no room, cached buffers go back to the device first,
+// then `reclaim` is asked to move a resident raster out of HBM, until the
+// allocation fits or nothing is left to move.
 class Store {
  public:
   void* alloc(size_t bytes);
   void release(void* p);
+  void discard(void* p);  // a buffer in use goes back to the device now
   void trim();        // returns cached buffers to the device
   size_t in_use() const { return in_use_; }
   size_t cached() const { return cached_; }
+  size_t cap = 0;     // bytes held (in use + cached) at most; 0: no cap
+  std::function<bool()> reclaim;  // frees some store memory; false when it cannot
   ~Store();
  private:
   std::multimap<size_t, void*> free_;
@@ -80,8 +92,11 @@ class Runtime {
   void setupDevices(const std::string& plat, unsigned dev_type, const std::string& dev_name);
   void evaluate(const std::vector<Node*>& targets);
 
-  // D2 node whose raster is resident in HBM (after evaluation, while referenced).
-  bool isResident(const Node* n) const { return resident_.count(const_cast<Node*>(n)) != 0; }
+  // D2 node whose raster is held after evaluation, while referenced: in HBM or, moved out for room, in host memory
+  bool isResident(const Node* n) const {
+    Node* m = const_cast<Node*>(n);
+    return resident_.count(m) != 0 || spilled_.count(m) != 0;
+  }
   void* residentPtr(Node* n);
   size_t rasterBytes(const Meta& m) const;
   static Node* radial_node(const Group& g);  // the RadialScan node of a radial group, or null
@@ -152,10 +167,11 @@ class Runtime {
   // One evaluation in progress: its plan, levels and the use counts of its rasters.
   struct Eval {
     std::vector<Node*> targets, list;
+    std::vector<Node*> tsorted;  // the targets sorted by address (d2_outputs)
     std::shared_ptr<Plan> plan;
     std::vector<std::vector<const Group*>> levels;
-    std::map<Node*, int> uses;
-    std::set<Node*> tset;
+    std::unordered_map<Node*, int> uses;
+    std::unordered_set<Node*> tset;
     // kernels fixed at planning time (requests of the server run later)
     std::unordered_map<const Group*, std::pair<std::vector<Node*>, KernelEntry*>> kern;
   };
@@ -166,6 +182,8 @@ class Runtime {
   void eval_finish(Eval& e);
   std::vector<Node*> kernel_outputs(const Group& g, const std::vector<Node*>& targets, bool& has_zonal);
   void run_levels(const std::vector<std::pair<Eval*, size_t>>& work, std::vector<Readback>* async);
+  static constexpr int kBigLevel = 64;    // levels of at least this many groups run unrecorded
+  static constexpr int kSubBatch = 512;   // groups prepared and launched together when unrecorded
   bool prepare_task(const Group& g, Eval& e, Task& t);
   void radial_task(const Group& g, Eval& e);
   void spread_task(const Group& g, Eval& e);
@@ -213,6 +231,9 @@ class Runtime {
     std::vector<int> param_node, d2_out, zout;  // local numbers of the spec's node references
   };
   std::unordered_map<std::string, ShapeEntry> shapes_;
+  unsigned visit_stamp_ = 0;  // visit marks of cut_list and reach (Node::vstamp, Node::tstamp)
+  std::vector<Node*> reach_, reach_stack_;
+  const std::vector<Node*>& reach(const std::vector<Node*>& roots);  // nodes upwards of roots, each once
   std::unordered_map<std::string, std::shared_ptr<Plan>> plans_;
   std::multimap<size_t, void*> pinned_free_;
   std::unordered_map<void*, size_t> pinned_size_;
@@ -226,6 +247,20 @@ class Runtime {
 
   std::unordered_map<Node*, void*> resident_;
   std::unordered_map<Node*, int> pins_;  // rasters read by admitted requests: not dropped
+  // Rasters moved out of HBM to pinned host memory when the store needed the
+  // room (least recently used first); they come back when next read.
+  struct Spilled {
+    void* host = nullptr;
+    bool generated = false;  // made by a generator: the host copy is its only source
+  };
+  std::unordered_map<Node*, Spilled> spilled_;
+  unsigned long long use_clock_ = 0;               // one tick per evaluation / admitted request
+  std::multiset<unsigned long long> live_ticks_;   // ticks of evaluations and requests in flight
+  bool spill_one();        // moves the least recently used idle raster out of HBM
+  void unspill(Node* n);   // brings a spilled raster back
+  void forget_spilled(Node* n);
+  unsigned long long tick_begin(const std::vector<Node*>& list);  // marks the nodes of a new evaluation
+  void tick_end(unsigned long long t);
   struct ServeState;
   struct Request;
   struct CacheEntry {

@@ -37,6 +37,7 @@ struct Runtime::Request {
   std::vector<Node*> computes;  // nodes this request computes
   std::vector<Node*> pinned;    // resident rasters it reads
   Request* blocker = nullptr;   // waiting: the admitted request computing a node it needs
+  unsigned long long tick = 0;  // while admitted: its nodes stay in HBM (Runtime::spill_one)
 };
 
 struct Runtime::ServeState {
@@ -106,6 +107,7 @@ void Runtime::serve_retire() {
     }
     for (Node* n : r.pinned)
       if (--pins_[n] == 0) pins_.erase(n);
+    tick_end(r.tick);
     for (Node* n : r.ev.targets) n->ref--;
     S.inflight -= r.footprint;
     S.done.push_back({r.id, r.t_submit, r.t_admit, t});
@@ -156,6 +158,9 @@ int Runtime::serveStep(int wait_us) {
     }
     size_t fp = 0;
     std::vector<Node*> pins;
+    // marked before any input is brought back from host memory, so that doing
+    // so never moves out another raster this request reads
+    const unsigned long long tick = tick_begin(e.list);
     try {
       for (auto& g : e.plan->groups) {
         if (host_group(*g)) continue;
@@ -164,15 +169,17 @@ int Runtime::serveStep(int wait_us) {
         if (rn || sn) {  // no generated kernel: its output raster (and a spread's count words)
           bool z = false;
           const Meta& m = (rn ? rn : sn)->meta;
-          if (!kernel_outputs(*g, e.targets, z).empty())
+          if (!kernel_outputs(*g, e.tsorted, z).empty())
             fp += rasterBytes(m) + (sn ? (size_t)m.bs.arr[0] * m.bs.arr[1] * m.nb(0) * m.nb(1) * 8 + 8 : 0);
-          for (Node* n : g->ins)
+          for (Node* n : g->ins) {
+            if (n->meta.nd == D2 && spilled_.count(n)) unspill(n);
             if (n->meta.nd == D2 && resident_.count(n)) pins.push_back(n);
             else if (n->meta.nd == D2 && n->kind == Kind::Read) fp += rasterBytes(n->meta);
+          }
           continue;
         }
         bool zonal = false;
-        std::vector<Node*> kouts = kernel_outputs(*g, e.targets, zonal);
+        std::vector<Node*> kouts = kernel_outputs(*g, e.tsorted, zonal);
         KernelEntry* ke = nullptr;
         if (!kouts.empty() || zonal) ke = &kernel_for(e.plan.get(), *g, kouts);
         e.kern[g.get()] = {kouts, ke};
@@ -180,19 +187,23 @@ int Runtime::serveStep(int wait_us) {
         if (zonal) fp += (size_t)(rasterBytes(g->meta()) / 4096 + 1) * ke->spec.part_bytes;
         for (Node* n : g->ins) {
           if (n->meta.nd != D2) continue;
+          if (spilled_.count(n)) unspill(n);                       // back in HBM for the request
           if (resident_.count(n)) pins.push_back(n);               // read, never dropped while admitted
           else if (n->kind == Kind::Read) fp += rasterBytes(n->meta);  // uploaded when first read
         }
       }
     } catch (...) {
+      tick_end(tick);
       eval_uncut(e);
       throw;
     }
     eval_uncut(e);
     if (!S.active.empty() && S.inflight + fp > S.budget) {
+      tick_end(tick);
       S.waits_memory++;
       break;  // memory frees up as requests retire
     }
+    r.tick = tick;
     r.ev = std::move(e);
     r.footprint = fp;
     r.t_admit = now_s();
