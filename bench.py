@@ -51,6 +51,101 @@ def cpu_baseline(s0_full, bs, budget_s=12.0):
                       "oracle/oracle.c or_life with OpenMP" % (side, side, gens, bs[0], bs[1])}
 
 
+def _cpu_timed(run, cells, budget_s):
+    """Calls run() (one pass over `cells` cells) until budget_s seconds have
+    passed (at least once); returns (Gcells/s, passes)."""
+    n, t0 = 0, time.perf_counter()
+    while True:
+        run()
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt >= budget_s:
+            return cells * n / dt / 1e9, n
+
+
+CPU_PORT_NOTE = ("the oracle (oracle/ref.py, FP32 semantics: focal taps in C with OpenMP over the host "
+                 "cores, the per-cell steps numpy)")
+
+
+def cpu_hillshade(full_side, side=2048, block=512, budget_s=4.0):
+    """The oracle's hillshade on the top-left side^2 corner of the hillshade workload's DEM."""
+    import numpy as np
+    from oracle import ref as O
+    from map_amd import workloads as W
+    rng = np.random.Generator(np.random.PCG64(0))
+    D = O.from_array(O.dem(0, 0, side, side, W.dem_gaussians(rng, full_side, full_side), 0), (block, block))
+    v, n = _cpu_timed(lambda: W.hillshade(O, D), side * side, budget_s)
+    return {"value": float("%.4g" % v), "unit": "Gcells/s", "cores": os.cpu_count() or 1, "kind": "port",
+            "sample": "hillshade of the %dx%d top-left corner of the same DEM, blocks %dx%d, %d passes, %s"
+                      % (side, side, block, block, n, CPU_PORT_NOTE)}
+
+
+def cpu_hillshade_zonal(full_side, side=2048, block=1024, budget_s=4.0):
+    """The oracle's hillshade + 256-zone statistics on the top-left corner of the workload's inputs."""
+    import numpy as np
+    from oracle import ref as O
+    from map_amd import workloads as W
+    rng = np.random.Generator(np.random.PCG64(2))
+    gauss = W.dem_gaussians(rng, full_side, full_side, ref_side=4096.0 * full_side / 4096)
+    rz = np.random.Generator(np.random.PCG64(3))
+    px, py = rz.integers(0, full_side, 256), rz.integers(0, full_side, 256)
+    D = O.from_array(O.dem(0, 0, side, side, gauss, 2), (block, block))
+    Z = O.from_array(O.voronoi(0, 0, side, side, px, py), (block, block))
+    v, n = _cpu_timed(lambda: W.hillshade_zonal(O, D, Z, 256), side * side, budget_s)
+    return {"value": float("%.4g" % v), "unit": "Gcells/s", "cores": os.cpu_count() or 1, "kind": "port",
+            "sample": "hillshade + 256-zone stats of the %dx%d top-left corner of the same DEM and zones, "
+                      "blocks %dx%d, %d passes, %s" % (side, side, block, block, n, CPU_PORT_NOTE)}
+
+
+def cpu_batched_tiles(tiles, block, budget_s=5.0):
+    """The oracle's per-tile pipeline over the first tiles of the same population
+    ((w, h, gaussians, dem seed, zone points, weights) per tile), until the budget."""
+    import numpy as np
+    from oracle import ref as O
+    from map_amd import workloads as W
+    cells, k, dt = 0, 0, 0.0
+    bs = (block, block)
+    for w, h, gauss, dseed, px, py, wts in tiles:  # inputs made outside the timed part
+        d = O.from_array(O.dem(0, 0, w, h, gauss, dseed), bs)
+        z = O.from_array(O.voronoi(0, 0, w, h, px, py).astype(np.uint8), bs)
+        t0 = time.perf_counter()
+        W.batch_tile_sums(O, d, z, wts)
+        dt += time.perf_counter() - t0
+        cells += w * h
+        k += 1
+        if dt >= budget_s:
+            break
+    return {"value": float("%.4g" % (cells / dt / 1e9)), "unit": "Gcells/s", "cores": os.cpu_count() or 1,
+            "kind": "port",
+            "sample": "the first %d tiles of the same population (%d cells), %s" % (k, cells, CPU_PORT_NOTE)}
+
+
+def cpu_serve_mix(block=1024, side=1024, budget_s=2.0):
+    """The oracle on each request kind of the served mix over the side^2 top-left corner
+    of DEM id 0 (zones for the zonal kind); the mix's rate weights each kind's time by
+    its share of the requests (every kind covers its DEM's cells)."""
+    import numpy as np
+    from oracle import ref as O
+    from map_amd import workloads as W
+    r = np.random.Generator(np.random.PCG64([7, 0]))
+    w, h = (int(round(np.exp(r.uniform(np.log(512), np.log(16384))))) for _ in range(2))
+    r = np.random.Generator(np.random.PCG64([8, 0]))
+    gauss = W.dem_gaussians(r, w, h)
+    px, py = r.integers(0, w, 64), r.integers(0, h, 64)
+    sw, sh = min(side, w), min(side, h)
+    D = O.from_array(O.dem(0, 0, sw, sh, gauss, 5000), (block, block))
+    Z = O.from_array(O.voronoi(0, 0, sw, sh, px, py), (block, block))
+    per = {}
+    for kind in SERVE_KINDS:
+        v, _ = _cpu_timed(lambda: W.serve_request(O, kind, D, Z if kind == "zstats" else None), sw * sh, budget_s)
+        per[kind] = float("%.4g" % v)
+    mix = 1.0 / sum(p / per[k] for k, p in zip(SERVE_KINDS, SERVE_MIX))
+    return {"value": float("%.4g" % mix), "unit": "Gcells/s", "cores": os.cpu_count() or 1, "kind": "port",
+            "per_kind_gcells_s": per,
+            "sample": "each request kind on the %dx%d top-left corner of DEM id 0, blocks %dx%d, the mix's "
+                      "shares as weights, %s" % (sw, sh, block, block, CPU_PORT_NOTE)}
+
+
 def _progress(msg):
     """A line on stderr per stage of the long sections."""
     print("bench: " + msg, file=sys.stderr, flush=True)
@@ -66,7 +161,7 @@ def _kernel_profile(M, fn):
     return kt
 
 
-def hillshade_zonal_bench(M, side=32768, block=1024, reps=3):
+def hillshade_zonal_bench(M, side=32768, block=1024, reps=3, cpu=True):
     """Hillshade + per-zone statistics over 256 Voronoi zones on a side^2 FP32
     DEM (seeded Gaussians + fBm) and an S32 zone raster, fused into one kernel
     (focal_zonal) and in the reference's grouping (hillshade materialised)."""
@@ -103,11 +198,13 @@ def hillshade_zonal_bench(M, side=32768, block=1024, reps=3):
     alg = 8.0 * cells  # DEM 4 B + zone 4 B per cell
     f["achieved_gbs"] = round(alg / (f["main_kernel_ms"] * 1e-3) / 1e9, 1)
     f["hbm_frac"] = round(f["achieved_gbs"] / HBM_PEAK_GBS, 4)
+    if cpu:
+        out["cpu_baseline"] = cpu_hillshade_zonal(side, min(side, 2048), block)
     return {"workload": "hillshade + per-zone stats (256 Voronoi zones), %dx%d f32 DEM, s32 zones, blocks %dx%d"
                         % (side, side, block, block), "algorithmic_bytes_per_cell": 8, **out}
 
 
-def batched_tiles_bench(M, ntiles=2048, block=1024, seed=4, forms=("batched", "plain")):
+def batched_tiles_bench(M, ntiles=2048, block=1024, seed=4, forms=("batched", "plain"), cpu=True):
     """2048 independent tiles of log-uniform sides in [256, 4096]: 5x5
     weighted focal sum -> slope -> mask slope > 0.5 -> per-zone sums of the mask
     over 16 Voronoi zones (U8). Batched (one launch per kernel over a device
@@ -115,14 +212,18 @@ def batched_tiles_bench(M, ntiles=2048, block=1024, seed=4, forms=("batched", "p
     import numpy as np
     from map_amd import workloads as W
     rng = np.random.Generator(np.random.PCG64(seed))
-    tiles, cells = [], 0
+    tiles, cells, specs = [], 0, []
     for i in range(ntiles):
         w, h = (int(round(np.exp(rng.uniform(np.log(256), np.log(4096))))) for _ in range(2))
         trng = np.random.Generator(np.random.PCG64([seed, i]))
-        dem = M.gen_dem((w, h), (block, block), W.dem_gaussians(trng, w, h), 1000 + i)
-        zones = M.gen_zones((w, h), (block, block), trng.integers(0, w, 16), trng.integers(0, h, 16), M.U8)
+        gauss = W.dem_gaussians(trng, w, h)
+        px, py = trng.integers(0, w, 16), trng.integers(0, h, 16)
+        dem = M.gen_dem((w, h), (block, block), gauss, 1000 + i)
+        zones = M.gen_zones((w, h), (block, block), px, py, M.U8)
         wts = (trng.integers(-64, 65, (5, 5)) / 64.0).tolist()
         tiles.append((dem, zones, wts))
+        if i < 64:
+            specs.append((w, h, gauss, 1000 + i, px, py, wts))
         cells += w * h
         if i % 256 == 255:
             _progress("batched tiles: %d tiles generated" % (i + 1))
@@ -157,7 +258,8 @@ def batched_tiles_bench(M, ntiles=2048, block=1024, seed=4, forms=("batched", "p
             res[name] = {"wall_ms": round(min(walls) * 1e3, 2), "device_ms": round(min(devs), 2),
                          "first_eval_ms": round(first * 1e3, 1),
                          "gcells_s": round(cells / min(walls) / 1e9, 2),
-                         "host_ms_last": {k: round(h1[k] - h0[k], 2) for k in ("ms_plan", "ms_fuse", "ms_codegen", "ms_prepare", "ms_launch")},
+                         "host_ms_last": {k: round(h1[k] - h0[k], 2) for k in ("ms_collect", "ms_predict", "ms_plan", "ms_fuse",
+                                                                               "ms_codegen", "ms_prepare", "ms_launch")},
                          "launches_last": h1["launches"] - h0["launches"],
                          "sums": [o.table()["sum"] for o in outs]}
             if name == "batched":  # per-kernel device time of one evaluation (launches timed, no graph)
@@ -176,6 +278,8 @@ def batched_tiles_bench(M, ntiles=2048, block=1024, seed=4, forms=("batched", "p
     for r in res.values():
         if isinstance(r, dict):
             r.pop("sums", None)
+    if cpu:
+        res["cpu_baseline"] = cpu_batched_tiles(specs, block)
     return {"workload": "%d tiles, sides log-uniform [256, 4096], 5x5 weighted sum -> slope -> mask -> 16-zone sums, "
                         "blocks %dx%d" % (ntiles, block, block),
             "cells": cells, "outputs_identical": bool(same) if both else None,
@@ -294,7 +398,7 @@ def sequential_run(M, cache, stream, form):
             "service_p99_ms": round(float(np.percentile(lat, 99)), 3)}
 
 
-def serve_bench(M, block=1024, seconds=5.0, seed=9):
+def serve_bench(M, block=1024, seconds=5.0, seed=9, cpu=True):
     """Served mix (40% hillshade, 30% slope+aspect, 20% 7x7 mean, 10% 64-zone
     stats; DEM sides log-uniform [512, 16384], 500 DEM ids, Zipf 1.1): the
     plain (per-block launches) and batched forms one request at a time, then
@@ -318,7 +422,48 @@ def serve_bench(M, block=1024, seconds=5.0, seed=9):
     cs = cache.stats()
     out["dem_cache"] = {"hits": cs["hits"], "misses": cs["misses"], "evictions": cs["evictions"],
                         "resident_gb": round(cs["bytes"] / 2**30, 1)}
+    if cpu:
+        out["cpu_baseline"] = cpu_serve_mix(block)
     return out
+
+
+def hillshade_parts_bench(M, side=4096, block=1024, nz=256):
+    """Per-zone statistics (256 Voronoi zones) of each prefix of the hillshade script, fused
+    into one kernel each (focal_zonal), to attribute a kernel's VALU instructions per node
+    (run under rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU): the DEM alone, the horizontal
+    gradient, the squared magnitude, its square root, the slope, cos / sin+cos of the slope,
+    the aspect, cos of the aspect term, the hillshade. Prints each part's kernel time."""
+    import numpy as np
+    from map_amd import workloads as W
+    rng = np.random.Generator(np.random.PCG64(2))
+    dem = M.gen_dem((side, side), (block, block), W.dem_gaussians(rng, side, side), 2)
+    rz = np.random.Generator(np.random.PCG64(3))
+    zones = M.gen_zones((side, side), (block, block), rz.integers(0, side, nz), rz.integers(0, side, nz))
+    def mag():
+        x, y = W.hori(M, dem), W.vert(M, dem)
+        return x * x + y * y
+
+    # built one at a time, so that no intermediate is held (and kept) by the caller
+    parts = [("dem", lambda: dem * 1.0), ("hori", lambda: W.hori(M, dem)), ("mag", mag),
+             ("sqrt", lambda: M.sqrt(mag())), ("slope", lambda: W.slope(M, dem)),
+             ("cos_slope", lambda: M.cos(W.slope(M, dem)) * 1.0),
+             ("sincos_slope", lambda: M.cos(W.slope(M, dem)) + M.sin(W.slope(M, dem))),
+             ("aspect", lambda: W.aspect(M, dem)), ("cos_aspect", lambda: M.cos(2.356 - W.aspect(M, dem))),
+             ("hillshade", lambda: W.hillshade(M, dem))]
+    M.set_option("focal_zonal", 1)
+    out = {}
+    try:
+        for name, make in parts:
+            zs = M.zstats(make(), zones, nz)
+            M.eval(zs)
+            kt = _kernel_profile(M, lambda: (M.release(zs), M.eval(zs)))
+            k, rec = max(kt.items(), key=lambda kv: kv[1]["total_ms"])
+            out[name] = {"kernel": k, "ms": round(rec["total_ms"] / rec["count"], 4)}
+            del zs
+    finally:
+        M.set_option("focal_zonal", 0)
+    return {"workload": "per-zone stats (%d Voronoi zones) of prefixes of the hillshade script, %dx%d f32 DEM, "
+                        "blocks %dx%d, one fused kernel each" % (nz, side, side, block, block), "parts": out}
 
 
 def radial_bench(M, side=16384, block=1024, reps=3):
@@ -398,7 +543,7 @@ def spread_bench(M, side=16384, block=1024, reps=3):
     return res
 
 
-def hillshade_bench(M, side=4096, block=512, reps=5):
+def hillshade_bench(M, side=4096, block=512, reps=5, cpu=True):
     """The plain hillshade script (python/hill.py) on a side^2 FP32 DEM in
     block^2 blocks: one fused focal+local group, plain and batched forms."""
     import numpy as np
@@ -422,6 +567,8 @@ def hillshade_bench(M, side=4096, block=512, reps=5):
                      "kernel_ms_per_launch": round(k["total_ms"] / k["count"], 4), "launches": k["count"]}
         del hs
     M.set_option("form", 1)
+    if cpu:
+        out["cpu_baseline"] = cpu_hillshade(side, min(side, 2048), block)
     return {"workload": "hillshade %dx%d f32, blocks %dx%d" % (side, side, block, block), **out}
 
 
@@ -441,7 +588,8 @@ def main():
     ap.add_argument("--tile", default="", help="TX,TY,VX[,PF] tile shape of the Life kernel (implies --tune 0)")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the Life headline (skip hillshade, hillshade+zonal, batched tiles, serving)")
-    ap.add_argument("--only", choices=["hillshade", "hillshade_zonal", "batched_tiles", "serve", "radial", "spread"],
+    ap.add_argument("--only", choices=["hillshade", "hillshade_zonal", "batched_tiles", "serve", "radial", "spread",
+                             "hillshade_parts"],
                     help="run one extra section alone (no Life headline), print its JSON")
     ap.add_argument("--c3-forms", default="batched,plain",
                     help="forms of the batched-tiles section (batched, plain), comma-separated")
@@ -457,8 +605,11 @@ def main():
     if args.only:
         fn = {"hillshade": hillshade_bench, "hillshade_zonal": hillshade_zonal_bench,
               "batched_tiles": batched_tiles_bench, "serve": serve_bench, "radial": radial_bench,
+              "hillshade_parts": hillshade_parts_bench,
               "spread": spread_bench}[args.only]
         kw = {"forms": tuple(args.c3_forms.split(","))} if args.only == "batched_tiles" else {}
+        if args.only in ("hillshade", "hillshade_zonal", "batched_tiles", "serve"):
+            kw["cpu"] = not args.no_cpu
         print(json.dumps({args.only: fn(M, **kw)}), flush=True)
         return
     bs = (args.block, args.block)
@@ -598,7 +749,8 @@ def main():
                          ("spread", spread_bench)):
             t0 = time.perf_counter()
             try:
-                extras[name] = fn(M)
+                extras[name] = fn(M, cpu=not args.no_cpu) if name in ("hillshade", "hillshade_zonal", "batched_tiles",
+                                                                       "serve") else fn(M)
             except Exception as e:  # reported, the headline stands
                 import traceback
                 traceback.print_exc(file=sys.stderr)
