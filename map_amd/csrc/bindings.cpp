@@ -425,7 +425,8 @@ const char* ma_runStats(void) {
   std::ostringstream s;
   s << "{\"evals\": " << r.cnt.evals << ", \"groups\": " << r.cnt.groups << ", \"launches\": " << r.cnt.launches
     << ", \"compiles\": " << r.cnt.compiles << ", \"shape_hits\": " << r.cnt.shape_hits
-    << ", \"predicted\": " << r.cnt.predicted << ", \"spills\": " << r.cnt.spills
+    << ", \"predicted\": " << r.cnt.predicted << ", \"narrow_launches\": " << r.cnt.narrow_launches
+    << ", \"spills\": " << r.cnt.spills
     << ", \"unspills\": " << r.cnt.unspills << ", \"dropped_for_space\": " << r.cnt.dropped_for_space
     << ", \"bytes_spilled\": " << r.cnt.bytes_spilled
     << ", \"scalar_tasks\": " << r.cnt.scalar_tasks << ", \"bytes_h2d\": " << r.cnt.bytes_h2d

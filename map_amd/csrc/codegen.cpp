@@ -337,6 +337,9 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     ks.stream = ok;
   }
   s << "#include \"mapx_dev.hiph\"\n";
+  // the module for rasters with an edge block narrower than the halo is the
+  // same source compiled with MX_NARROW 1 (narrow_source)
+  s << "#ifndef MX_NARROW\n#define MX_NARROW 0\n#endif\n";
   s << "namespace {\nstruct P {\n";
   s << "  static constexpr int H0 = " << ks.H0 << ", H1 = " << ks.H1 << ", PAD = " << PAD << ", SW = " << SW << ";\n";
   s << "  struct Args {\n";
@@ -363,9 +366,9 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     s << "  template <int VX> struct Span {\n";
     for (Node* st : staged) s << "    " << CT(st->meta.dt) << " s" << LN(st) << "[VX + " << 2 * span_pad[st] << "];\n";
     s << "  };\n";
-    s << "  template <int VX> __device__ static __forceinline__ void span_row(const Args& a, const mx::Blk& k, int x, int y, const mx::RowSrc& rs, Span<VX>& sp) {\n";
+    s << "  template <int VX, bool NW> __device__ static __forceinline__ void span_row(const Args& a, const mx::Blk& k, int x, int y, const mx::RowSrc& rs, Span<VX>& sp) {\n";
     for (Node* st : staged)
-      s << "    mx::ld_span<" << CT(st->meta.dt) << ", VX, " << span_pad[st] << ", H0>(a.i" << LN(st) << ", k, x, y, rs, sp.s" << LN(st) << ");\n";
+      s << "    mx::ld_span<" << CT(st->meta.dt) << ", VX, " << span_pad[st] << ", H0, NW>(a.i" << LN(st) << ", k, x, y, rs, sp.s" << LN(st) << ");\n";
     s << "  }\n";
     s << "  template <int VX> __device__ static __forceinline__ void span_zero(Span<VX>& sp) {\n";
     for (Node* st : staged)
@@ -555,12 +558,12 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     s << "    mx::ldpack<" << CT(n->meta.dt) << ", VX>(a.i" << LN(n) << ", rs, x0v, r.b" << LN(n) << ");\n";
   s << "    (void)a; (void)rs; (void)x0v; (void)r;\n  }\n";
   auto pre_body = [&](const char* fname, bool put) {
-    s << "  template <int VX> __device__ static __forceinline__ void " << fname << "(const Args& a, const mx::Blk& k, "
-      << (put ? "const Raw<VX>& r, " : "") << "int x0v, int y, Smem& sm, int si) {\n";
+    s << "  template <int VX" << (put ? "" : ", bool NW") << "> __device__ static __forceinline__ void " << fname
+      << "(const Args& a, const mx::Blk& k, " << (put ? "const Raw<VX>& r, " : "") << "int x0v, int y, Smem& sm, int si) {\n";
     for (Node* n : raw_in) {
       s << "    " << CT(n->meta.dt) << " b" << LN(n) << "[VX];\n";
       if (put) s << "    r.b" << LN(n) << ".unpack(b" << LN(n) << ");\n";
-      else s << "    mx::ldv<" << CT(n->meta.dt) << ", VX, H0>(a.i" << LN(n) << ", k, x0v, y, b" << LN(n) << ");\n";
+      else s << "    mx::ldv<" << CT(n->meta.dt) << ", VX, H0, NW>(a.i" << LN(n) << ", k, x0v, y, b" << LN(n) << ");\n";
     }
     for (Node* st : staged) s << "    " << CT(st->meta.dt) << " t" << LN(st) << "[VX];\n";
     s << "#pragma unroll\n    for (int v = 0; v < VX; v++) {\n      const int x = x0v + v;\n";
@@ -704,7 +707,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   s << "  __shared__ P::Smem sm;\n  __shared__ int slot;\n";
   s << "  mx::run_chunks<P, " << tile.TX << ", " << tile.TY << ", " << tile.VX << ", "
     << (ks.form != Form::Focal ? "mx::LOCAL" : ks.stream ? "mx::STREAM" : "mx::STAGED") << ", " << (tile.PF ? "true" : "false")
-    << ">(a0, L0, tasks, starts, ntasks, nchunks, queue, zpart, acc, sm, &slot);\n";
+    << ", (MX_NARROW != 0)>(a0, L0, tasks, starts, ntasks, nchunks, queue, zpart, acc, sm, &slot);\n";
   s << "}\n";
 
   // fold kernel: folds partial records in index order. Work-group (f, t) folds

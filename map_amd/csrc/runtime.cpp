@@ -1346,6 +1346,21 @@ void Runtime::run_batch(const std::vector<Task*>& batch, std::vector<Readback>* 
   dev::Function fn;
   fn.fn = ke.fn;
   fn.name = ks.name;
+  // a raster whose last block column or row is narrower than the halo needs the
+  // stepwise border rule: the batch runs the module compiled for it (it reads
+  // the other rasters of the batch alike)
+  if (ks.H0 > 0 || ks.H1 > 0) {
+    bool narrow = false;
+    for (const Task* t : batch) {
+      const HostLaunch& L = t->L;
+      narrow |= L.DS0 - (L.NB0 - 1) * L.BS0 < ks.H0 || L.DS1 - (L.NB1 - 1) * L.BS1 < ks.H1;
+    }
+    if (narrow) {
+      HostTimer ht(cnt.ms_codegen);
+      fn.fn = dev::function("#define MX_NARROW 1\n" + ks.source, ks.name).fn;
+      cnt.narrow_launches++;
+    }
+  }
   // persistent grid: every work-group resident at once (no tail wave), a
   // multiple of 8 for the XCD-contiguous split
   // work-groups per CU: the occupancy query's answer, or the option's value

@@ -46,6 +46,7 @@ struct Counters {
   long evals = 0, groups = 0, launches = 0, scalar_tasks = 0, compiles = 0;
   long shape_hits = 0;  // kernels taken from the shape cache instead of generated
   long predicted = 0;   // zonal reductions of constant rasters known without a kernel
+  long narrow_launches = 0;  // batches run by the module for edge blocks narrower than the halo
   long bytes_h2d = 0, bytes_d2h = 0;
   long spills = 0, unspills = 0, dropped_for_space = 0;  // resident rasters moved out of HBM and back
   long bytes_spilled = 0;

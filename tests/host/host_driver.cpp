@@ -89,6 +89,17 @@ void fetch(const R& r) {
   if (ma_fetch(r.n, h.data(), n) != 0) fail("fetch");
 }
 
+// a raster whose last block column and row are narrower than a 5x5 window's halo
+void scenario_narrow(std::mt19937& rng) {
+  R d = input(129, 65, 128, 64, rng);
+  ma_variant_t w[25];
+  for (int i = 0; i < 25; i++) w[i] = s32(1 + i % 3);
+  R c = R(ma_convolution(d.n, w, 25, dim(5, 5)), "convolution");
+  eval({c});
+  fetch(c);
+  if (!std::strstr(ma_runStats(), "\"narrow_launches\": 0,") == false) fail("the narrow module (not used)");
+}
+
 void scenario_lifetimes(std::mt19937& rng) {
   for (int rep = 0; rep < 3; rep++) {
     R d = input(300, 200, 128, 64, rng);
@@ -255,6 +266,7 @@ int main() {
   ma_setupDevices("", DEV_GPU, "");
   std::mt19937 rng(7);
   scenario_lifetimes(rng);
+  scenario_narrow(rng);
   scenario_batched(rng, false);
   scenario_batched(rng, true);
   scenario_chain(rng);

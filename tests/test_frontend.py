@@ -34,6 +34,22 @@ def test_value_numbering(M):
     assert (a + 1).nodeid() != (a + 1.0).nodeid()
 
 
+def test_front_end_constant_and_mask_reuse_keeps_distinct_values_apart(M):
+    """Scalar operands and masks are reused by the front end per value; values that compare
+    equal but differ in type or sign (1 / 1.0 / True, 0.0 / -0.0) stay distinct nodes."""
+    r = dem(M)
+    ids = lambda *xs: [x._node.value for x in xs]
+    a, b, c = r * 1, r * 1.0, r * True
+    assert len(set(ids(a, b, c))) == 3
+    assert (r * 1.0)._node.value == b._node.value
+    assert (r * 0.0)._node.value != (r * -0.0)._node.value
+    assert (r * -0.0).datatype() == M.F32
+    m1, m2 = M.convolve(r, [[1, 0, 1]]), M.convolve(r, [[1.0, 0, 1]])
+    assert m1._node.value != m2._node.value
+    assert M.convolve(r, [[1, 0, 1]])._node.value == m1._node.value
+    assert M.convolve(r, [[1], [0], [1]])._node.value != m1._node.value  # same values, other shape
+
+
 def test_errors_raise(M):
     a = dem(M, 64, 48)
     b = dem(M, 32, 48)
@@ -118,7 +134,7 @@ def test_tile_fits_the_block(M):
     for side, bs in [(96, 48), (1024, 1024), (64, 16)]:
         u = M.from_array(np.zeros((side, side), np.uint8), (bs, bs))
         src = M.plan(scripts.life(M, u, 1))["groups"][0]["source"]
-        tx, ty, vx = map(int, re.search(r"run_chunks<P, (\d+), (\d+), (\d+), mx::(?:STAGED|STREAM), (?:true|false)>", src).groups())
+        tx, ty, vx = map(int, re.search(r"run_chunks<P, (\d+), (\d+), (\d+), mx::(?:STAGED|STREAM), (?:true|false), ", src).groups())
         assert tx <= bs and ty <= bs and 256 % (tx // vx) == 0, (bs, tx, ty, vx)
 
 

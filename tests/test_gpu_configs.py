@@ -316,6 +316,7 @@ def test_edge_blocks_narrower_than_the_halo(G, O, ds, bs):
     ref_mean = W.focal_mean7(O, O.from_array(f, bs)).val
     ref_k5 = O.convolve(O.from_array(u, bs), k5).val
     ref_life = O.life(life0, bs, 5)
+    n0 = G.stats()["narrow_launches"]
     try:
         for st in (0, 1):
             G.set_option("tile_st", st)
@@ -324,6 +325,7 @@ def test_edge_blocks_narrower_than_the_halo(G, O, ds, bs):
             assert_exact(G.to_array(W.life(G, G.from_array(life0, bs), 5)), ref_life, "life st=%d" % st)
     finally:
         G.set_option("tile_st", 0)
+    assert G.stats()["narrow_launches"] > n0  # the module compiled for narrow edge blocks ran
 
 
 def test_predicted_reductions_of_constant_rasters(G, O):
