@@ -286,12 +286,28 @@ class Raster:
         return astype(self, dt)
 
 
+
+
+# Scalar operands become D0 constant nodes; the backend's value numbering returns
+# the same node for the same constant, so the front end keeps one Raster per
+# constant instead of building a Variant and calling ma_constant each time (the
+# per-request cost of a served script is mostly these calls). Bounded.
+_CONSTS = {}
+
+
 def _const_to_raster(arg):
     if isinstance(arg, Raster):
         return arg
     if isinstance(arg, (bool, int, float)):
-        var = Variant(arg)
-        return Raster(_lib.ma_constant(var, Array(), var.datatype(), ROW + BLK, Array()))
+        key = repr(arg)  # tells apart what == does not: 1 / 1.0 / True, 0.0 / -0.0; NaN equals itself
+        r = _CONSTS.get(key)
+        if r is None:
+            var = Variant(arg)
+            r = Raster(_lib.ma_constant(var, Array(), var.datatype(), ROW + BLK, Array()))
+            if len(_CONSTS) >= 4096:
+                _CONSTS.clear()
+            _CONSTS[key] = r
+        return r
     raise TypeError("cannot use %r as a raster" % (arg,))
 
 
@@ -458,13 +474,23 @@ def con(cond, lhs, rhs):
     return Raster(_lib.ma_conditional(_const_to_raster(cond), _const_to_raster(lhs), _const_to_raster(rhs)))
 
 
+_MASKS = {}  # mask values -> the ctypes arguments (read only by the backend), bounded
+
+
 def _mask(lst):
     if not lst or not all(len(e) == len(lst[0]) for e in lst):
         raise ValueError("a mask is a non-empty list of equal-length rows")
-    ds = Array([len(lst[0]), len(lst)])
-    flat = [e for row in lst for e in row]
-    arr = (Variant * len(flat))(*[Variant(e) for e in flat])
-    return ct.cast(arr, ct.POINTER(Variant)), len(flat), ds, arr
+    key = repr(lst)  # shape and values, 1 / 1.0 / True and 0.0 / -0.0 apart
+    m = _MASKS.get(key)
+    if m is None:
+        flat = [e for row in lst for e in row]
+        ds = Array([len(lst[0]), len(lst)])
+        arr = (Variant * len(flat))(*[Variant(e) for e in flat])
+        m = (ct.cast(arr, ct.POINTER(Variant)), len(flat), ds, arr)
+        if len(_MASKS) >= 1024:
+            _MASKS.clear()
+        _MASKS[key] = m
+    return m
 
 
 def convolve(raster, lst):
