@@ -341,35 +341,53 @@ def serve_run(M, cache, stream, max_inflight=64, drain_s=20.0):
     lat, cells, nbytes = [], 0, 0
     arr = {}
     i, n = 0, len(stream)
+    t_build = t_submit = t_step = 0.0  # host seconds: building request scripts, submitting, stepping
+    h0 = M.stats()
     t0 = time.monotonic()
     try:
         while len(lat) < n:
             now = time.monotonic() - t0
-            while i < n and stream[i][0] <= now:
+            burst = 0  # the client and the server share this thread: the server steps between bursts
+            while i < n and stream[i][0] <= now and burst < max_inflight:
+                burst += 1
                 ta, kind, k = stream[i]
+                tb = time.perf_counter()
                 dem, z, _, c = cache.get(k, kind == "zstats")
-                rid = srv.submit(*W.serve_request(M, kind, dem, z))
+                outs = W.serve_request(M, kind, dem, z)
+                ts = time.perf_counter()
+                rid = srv.submit(*outs)
+                t_build += ts - tb
+                t_submit += time.perf_counter() - ts
                 arr[rid] = ta
                 cells += c
                 nbytes += c * SERVE_BYTES[kind]
                 i += 1
             wait = int(min(2000, max(0, (stream[i][0] - (time.monotonic() - t0)) * 1e6))) if i < n else 2000
-            for rid, ts, tadm, td in srv.step(wait):
+            ts = time.perf_counter()
+            done = srv.step(wait)
+            t_step += time.perf_counter() - ts
+            for rid, ts_, tadm, td in done:
                 lat.append(td - t0 - arr.pop(rid))
                 srv.result(rid)
             if i >= n and time.monotonic() - t0 > stream[-1][0] + drain_s:
                 break
         wall = time.monotonic() - t0
         st = srv.stats()
+        h1 = M.stats()
     finally:
         srv.close()
+    per = 1e3 / max(1, n)  # ms per request
+    host = {"build_script_ms": round(t_build * per, 4), "submit_ms": round(t_submit * per, 4),
+            "step_ms": round(t_step * per, 4)}
+    for key in ("ms_plan", "ms_fuse", "ms_codegen", "ms_prepare", "ms_launch"):
+        host[key[3:] + "_ms"] = round((h1[key] - h0[key]) / max(1, n), 4)  # inside step
     lat = np.array(lat) * 1e3
     return {"requests": n, "completed": int(len(lat)), "wall_s": round(wall, 3),
             "req_s": round(len(lat) / wall, 1), "gcells_s": round(cells / wall / 1e9, 2),
             "p50_ms": round(float(np.percentile(lat, 50)), 3) if len(lat) else None,
             "p99_ms": round(float(np.percentile(lat, 99)), 3) if len(lat) else None,
             "mean_tasks_per_step": round(st["group_tasks"] / max(1, st["steps"]), 2),
-            "waits_memory": st["waits_memory"]}
+            "waits_memory": st["waits_memory"], "host_ms_per_request": host}
 
 
 def sequential_run(M, cache, stream, form):
@@ -403,6 +421,18 @@ def serve_bench(M, block=1024, seconds=5.0, seed=9, cpu=True):
     stats; DEM sides log-uniform [512, 16384], 500 DEM ids, Zipf 1.1): the
     plain (per-block launches) and batched forms one request at a time, then
     the server at Poisson rates around and above the sequential capacity."""
+    import gc
+    # request scripts are acyclic (reference counting frees them): the cyclic
+    # collector's pauses would only add to the latencies measured
+    gc.collect()
+    gc.disable()
+    try:
+        return _serve_bench(M, block, seconds, seed, cpu)
+    finally:
+        gc.enable()
+
+
+def _serve_bench(M, block, seconds, seed, cpu):
     cache = DemCache(M, block, 96 << 30)
     warm = serve_stream(600, 0, seed + 1)
     _progress("serve: warming the DEM cache")
