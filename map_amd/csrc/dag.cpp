@@ -142,19 +142,33 @@ std::string Node::signature(bool float_weights) const {
 
 // ---- Dag ---------------------------------------------------------------------
 
+// Value-numbering key: the node's kind, inputs (by id), type, geometry and
+// parameters as raw bytes (every field a node kind uses; the others keep
+// their defaults, so they never tell equal nodes apart).
 std::string Dag::key(const Node* n) const {
-  std::ostringstream s;
-  s << (int)n->kind << "|";
-  for (auto* p : n->prev) s << p->id << ",";
-  s << "|" << n->meta.dt << "|" << n->meta.nd << "|" << n->meta.ds.arr[0] << "x" << n->meta.ds.arr[1] << "|"
-    << n->meta.bs.arr[0] << "x" << n->meta.bs.arr[1] << "|";
+  std::string k;
+  k.reserve(48 + 4 * n->prev.size() + 8 * n->mask.v.size() + n->path.size());
+  auto i32 = [&](int v) { k.append((const char*)&v, sizeof v); };
+  auto u64 = [&](uint64_t v) { k.append((const char*)&v, sizeof v); };
+  i32((int)n->kind);
+  i32((int)n->prev.size());
+  for (auto* p : n->prev) i32(p->id);
+  i32(n->meta.dt); i32(n->meta.nd);
+  i32(n->meta.ds.arr[0]); i32(n->meta.ds.arr[1]); i32(n->meta.bs.arr[0]); i32(n->meta.bs.arr[1]);
   switch (n->kind) {
-    case Kind::Constant: s << n->cnst.dtype << ":" << variant_bits(n->cnst); break;
-    case Kind::Read: s << (n->host ? "mem:" + std::to_string(n->host->uid) : n->path); break;
-    case Kind::Write: s << n->path; break;
-    default: s << n->signature(); break;
+    case Kind::Constant: i32(n->cnst.dtype); u64(variant_bits(n->cnst)); break;
+    case Kind::Read:
+      if (n->host) { k.push_back('m'); u64(n->host->uid); }
+      else { k.push_back('f'); k += n->path; }
+      break;
+    case Kind::Write: k += n->path; break;
+    default:
+      i32(n->op); i32(n->coord.arr[0]); i32(n->coord.arr[1]); i32(n->coord.n); i32(n->nzones);
+      i32(n->mask.ds.arr[0]); i32(n->mask.ds.arr[1]); i32((int)n->mask.v.size());
+      for (auto& w : n->mask.v) { i32(w.dtype); u64(variant_bits(w)); }
+      break;
   }
-  return s.str();
+  return k;
 }
 
 Node* Dag::add(std::unique_ptr<Node> n) {
