@@ -387,7 +387,7 @@ def serve_run(M, cache, stream, max_inflight=64, drain_s=20.0):
             "p50_ms": round(float(np.percentile(lat, 50)), 3) if len(lat) else None,
             "p99_ms": round(float(np.percentile(lat, 99)), 3) if len(lat) else None,
             "mean_tasks_per_step": round(st["group_tasks"] / max(1, st["steps"]), 2),
-            "waits_memory": st["waits_memory"], "host_ms_per_request": host}
+            "waits_memory": st["waits_memory"], "done_without_work": st["done_held"], "host_ms_per_request": host}
 
 
 def sequential_run(M, cache, stream, form):

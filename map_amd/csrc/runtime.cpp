@@ -1458,10 +1458,12 @@ void Runtime::run_levels(const std::vector<std::pair<Eval*, size_t>>& work, std:
   groups.reserve(n);
   for (auto& w : work)
     for (const Group* gp : w.first->levels[w.second]) groups.push_back({w.first, gp});
-  const size_t sub = dev::recording() || opt.form == FORM_PLAIN ? groups.size() : (size_t)kSubBatch;
+  const bool whole = dev::recording() || opt.form == FORM_PLAIN;
   size_t i = 0;
-  for (size_t g0 = 0; g0 < groups.size(); g0 += sub) {
-    const size_t g1 = std::min(groups.size(), g0 + sub);
+  for (size_t g0 = 0, g1 = 0; g0 < groups.size(); g0 = g1) {
+    // a short first sub-batch starts the device early, the rest go kSubBatch at a time
+    const size_t sub = whole ? groups.size() : (g0 == 0 ? (size_t)kFirstSubBatch : (size_t)kSubBatch);
+    g1 = std::min(groups.size(), g0 + sub);
     std::vector<Task*> order;
     {
       HostTimer ht_prep(cnt.ms_prepare);

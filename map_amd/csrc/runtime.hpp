@@ -185,6 +185,7 @@ class Runtime {
   void run_levels(const std::vector<std::pair<Eval*, size_t>>& work, std::vector<Readback>* async);
   static constexpr int kBigLevel = 64;    // levels of at least this many groups run unrecorded
   static constexpr int kSubBatch = 512;   // groups prepared and launched together when unrecorded
+  static constexpr int kFirstSubBatch = 64;  // the first of them
   bool prepare_task(const Group& g, Eval& e, Task& t);
   void radial_task(const Group& g, Eval& e);
   void spread_task(const Group& g, Eval& e);

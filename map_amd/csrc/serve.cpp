@@ -59,6 +59,7 @@ struct Runtime::ServeState {
   std::unordered_map<Node*, Request*> owner;
   std::vector<ServeDone> done;
   long admitted = 0, completed = 0, launch_steps = 0, tasks = 0, waits_memory = 0, waits_owner = 0;
+  long done_held = 0;  // requests whose targets were all held at admission (no work)
   size_t peak_inflight = 0;
 };
 
@@ -140,6 +141,28 @@ int Runtime::serveStep(int wait_us) {
   for (auto it = S.queued.begin(); it != S.queued.end();) {
     if ((int)S.active.size() >= S.max_inflight) break;
     Request& r = **it;
+    {
+      // targets already held (computed by an earlier request: value numbering
+      // gives identical requests the same nodes) need no work; their hold is
+      // released now, and a request left with none is done without planning
+      std::vector<Node*> left;
+      for (Node* n : r.ev.targets) {
+        const bool held = (n->meta.nd == D0 && n->evaluated && n->kind != Kind::Constant) ||
+                          (n->meta.nd == D2 && n->kind != Kind::Read && (resident_.count(n) || spilled_.count(n)));
+        if (held) n->ref--;
+        else left.push_back(n);
+      }
+      r.ev.targets = std::move(left);
+      if (r.ev.targets.empty()) {
+        const double t = now_s();
+        S.done.push_back({r.id, r.t_submit, t, t});
+        S.completed++;
+        S.admitted++;
+        S.done_held++;
+        it = S.queued.erase(it);
+        continue;
+      }
+    }
     Eval e;
     e.targets = r.ev.targets;
     eval_plan(e);
@@ -285,7 +308,7 @@ std::string Runtime::serveStats() {
     << ", \"completed\": " << S.completed << ", \"steps\": " << S.launch_steps << ", \"group_tasks\": " << S.tasks
     << ", \"inflight_bytes\": " << S.inflight << ", \"peak_inflight_bytes\": " << S.peak_inflight
     << ", \"budget_bytes\": " << S.budget << ", \"waits_memory\": " << S.waits_memory
-    << ", \"waits_owner\": " << S.waits_owner << "}";
+    << ", \"waits_owner\": " << S.waits_owner << ", \"done_held\": " << S.done_held << "}";
   return s.str();
 }
 

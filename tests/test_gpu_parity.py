@@ -301,6 +301,35 @@ def test_serve_results_same_alone_and_in_mix(G, O):
             assert np.array_equal(got[ids[i]][0]["min"], ref["min"])
 
 
+def test_serve_request_with_held_results_completes_without_work(G, O):
+    """A request whose targets are the nodes of an earlier request that still holds them
+    (value numbering gives identical scripts over the same DEM the same nodes) completes at
+    admission without planning or launching, with the same results."""
+    bs = (256, 256)
+    dem = G.from_array(dem_array(700, 500, 51), bs)
+    srv = G.Server(hbm_budget=64 << 20, max_inflight=4)
+    try:
+        first = scripts.serve_request(G, "slope_aspect", dem)
+        r1 = srv.submit(*first)
+        done = {}
+        while r1 not in done:
+            done.update({rid: 1 for rid, *_ in srv.step(2000)})
+        kept = srv.result(r1)  # the caller still holds the results
+        ref = [G.to_array(x) for x in kept]
+        l0 = G.stats()["launches"]
+        r2 = srv.submit(*scripts.serve_request(G, "slope_aspect", dem))
+        while r2 not in done:
+            done.update({rid: 1 for rid, *_ in srv.step(2000)})
+        again = srv.result(r2)
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st["done_held"] == 1 and G.stats()["launches"] == l0
+    for a, b in zip(again, ref):
+        assert_exact(G.to_array(a), b, "held request result")
+    del kept
+
+
 def test_sin_cos_pair_within_4ulp(G, O):
     """sin and cos of one value in one group share one sincos call; each
     result stays within 4 ulp of the correctly rounded value."""
