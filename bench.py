@@ -615,7 +615,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-gens", type=int, default=8, help="generations timed once in the plain form")
     ap.add_argument("--tune", type=int, default=1, help="autotune the tile shape before timing")
-    ap.add_argument("--tile", default="", help="TX,TY,VX[,PF] tile shape of the Life kernel (implies --tune 0)")
+    ap.add_argument("--tile", default="",
+                    help="TX,TY,VX[,PF[,ST[,W]]] tile shape of the Life kernel (implies --tune 0)")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the Life headline (skip hillshade, hillshade+zonal, batched tiles, serving)")
     ap.add_argument("--only", choices=["hillshade", "hillshade_zonal", "batched_tiles", "serve", "radial", "spread",
@@ -661,14 +662,14 @@ def main():
     tuned = None
     if args.tile:
         tv = [int(t) for t in args.tile.split(",")]
-        for key, val in zip(("tile_tx", "tile_ty", "tile_vx", "tile_pf"), tv):
+        for key, val in zip(("tile_tx", "tile_ty", "tile_vx", "tile_pf", "tile_st", "tile_w"), tv):
             M.set_option(key, val)
         args.tune = 0
     if args.tune:
         # the tile shape is tuned on a 10-generation script over the same raster:
         # every generation is the same fused group, so the choice carries over
         tune_out = life_script(M, S, 10)
-        rep = M.autotune(tune_out, reps=2)
+        rep = M.autotune(tune_out, reps=5)
         tuned = rep["chosen"]
         del tune_out
 

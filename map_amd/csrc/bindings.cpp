@@ -340,6 +340,10 @@ int ma_setOption(const char* key, long long value) {
     else if (k == "tile_vx") o.tile.VX = (int)value;
     else if (k == "tile_pf") o.tile.PF = value != 0;
     else if (k == "tile_st") o.tile.ST = value != 0;
+    else if (k == "tile_w") {
+      if (value < 0 || value > 8) throw std::runtime_error("setOption: tile_w (wavefronts per SIMD) in [0, 8]");
+      o.tile.W = (int)value;
+    }
     else if (k == "libm") o.libm = value != 0;
     else if (k == "predict") o.predict = value != 0;
     else if (k == "hbm_cap") {
@@ -375,6 +379,7 @@ long long ma_getOption(const char* key) {
   if (k == "tile_vx") return o.tile.VX;
   if (k == "tile_pf") return o.tile.PF;
   if (k == "tile_st") return o.tile.ST;
+  if (k == "tile_w") return o.tile.W;
   if (k == "libm") return o.libm;
   if (k == "predict") return o.predict;
   if (k == "hbm_cap") return (long long)o.hbm_cap;

@@ -161,7 +161,7 @@ std::string kernel_shape_key(const Group& g, const std::vector<Node*>& d2_outs, 
   for (size_t i = 0; i < all.size(); i++) local[all[i]] = (int)i;
   std::set<const Node*> in_set(g.ins.begin(), g.ins.end());
   std::ostringstream k;
-  k << tile.TX << "," << tile.TY << "," << tile.VX << "," << tile.PF << "," << tile.ST << "," << libm << "|";
+  k << tile.TX << "," << tile.TY << "," << tile.VX << "," << tile.PF << "," << tile.ST << "," << tile.W << "," << libm << "|";
   for (Node* n : all) {
     if (in_set.count(n)) { k << "I" << n->meta.nd << "." << n->meta.dt << ";"; continue; }
     k << n->signature(false) << "(";
@@ -684,7 +684,8 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   s << "}  // namespace\n\n";
 
   // main kernel
-  s << "extern \"C\" __global__ void __launch_bounds__(256) " << ks.name
+  s << "extern \"C\" __global__ void __launch_bounds__(256" << (tile.W > 0 ? ", " + std::to_string(tile.W) : std::string())
+    << ") " << ks.name
     << "(P::Args a0, mx::Launch L0, const mx::TaskRec<P::Args>* tasks, const int* starts, int ntasks, int nchunks, "
        "mx::u32* queue, char* zpart) {\n";
   s << "  P::Acc acc;\n";

@@ -40,8 +40,9 @@ struct TileShape {
   int TX = 64, TY = 16, VX = 1;
   int PF = 0;  // focal groups: load the next tile's rows while computing the current one
   int ST = 0;  // focal groups that stage only inputs: rows in registers (stream_tile) instead of LDS
+  int W = 0;   // wavefronts per SIMD the kernel's registers must allow (0: as the compiler allocates)
   bool operator==(const TileShape& o) const {
-    return TX == o.TX && TY == o.TY && VX == o.VX && PF == o.PF && ST == o.ST;
+    return TX == o.TX && TY == o.TY && VX == o.VX && PF == o.PF && ST == o.ST && W == o.W;
   }
 };
 

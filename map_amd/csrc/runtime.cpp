@@ -1503,7 +1503,7 @@ std::string Runtime::plan_key(const std::vector<Node*>& list, const std::vector<
   std::string k;
   k.reserve(list.size() * 9 + targets.size() * 8 + 16);
   auto put = [&](int v) { k.append((const char*)&v, sizeof v); };
-  put(opt.code_fusion); put(opt.focal_zonal); put(opt.tile.TX); put(opt.tile.TY); put(opt.tile.VX); put(opt.tile.PF); put(opt.tile.ST); put(opt.libm);
+  put(opt.code_fusion); put(opt.focal_zonal); put(opt.tile.TX); put(opt.tile.TY); put(opt.tile.VX); put(opt.tile.PF); put(opt.tile.ST); put(opt.tile.W); put(opt.libm);
   for (Node* n : list) { put(n->id); k.push_back(n->cut ? 'c' : (n->isOutput() ? 'o' : 'n')); }
   k.push_back('|');
   for (Node* t : targets) put(t->id);
@@ -1826,8 +1826,21 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   std::ostringstream rep;
   rep << "{\"candidates\": [";
   bool first = true;
+  // second pass: the best shapes again with the registers capped for 6 or 8
+  // wavefronts per SIMD (more work-groups per CU; a kernel that would spill loses)
+  std::vector<TileShape> pass(std::begin(cands), std::end(cands));
   try {
-    for (const TileShape& c : cands) {
+   for (int stage = 0; stage < 2; stage++) {
+    if (stage == 1) {
+      pass.clear();
+      for (auto& b : best)
+        for (int w : {6, 8}) {
+          TileShape c = b.second.second;
+          c.W = w;
+          if (std::find(pass.begin(), pass.end(), c) == pass.end()) pass.push_back(c);
+        }
+    }
+    for (const TileShape& c : pass) {
       opt.tile = c;
       for (Node* t : targets) drop(t);
       evaluate(targets);  // compile and warm
@@ -1845,11 +1858,12 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
         const double avg = k.total_ms / k.count;
         auto b = best.find(kg->second);
         if (b == best.end() || avg < b->second.first) best[kg->second] = {avg, c};
-        rep << (first ? "" : ", ") << "{\"tile\": [" << c.TX << ", " << c.TY << ", " << c.VX << ", " << c.PF << ", " << c.ST << "], \"kernel\": \""
+        rep << (first ? "" : ", ") << "{\"tile\": [" << c.TX << ", " << c.TY << ", " << c.VX << ", " << c.PF << ", " << c.ST << ", " << c.W << "], \"kernel\": \""
             << k.name << "\", \"avg_ms\": " << avg << "}";
         first = false;
       }
     }
+   }
   } catch (...) {
     opt.tile = saved;
     opt.graph = saved_graph;
@@ -1864,7 +1878,8 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   for (auto& b : best) {
     tuned_[b.first] = b.second.second;
     rep << (first ? "" : ", ") << "{\"avg_ms\": " << b.second.first << ", \"tile\": [" << b.second.second.TX << ", "
-        << b.second.second.TY << ", " << b.second.second.VX << ", " << b.second.second.PF << ", " << b.second.second.ST << "]}";
+        << b.second.second.TY << ", " << b.second.second.VX << ", " << b.second.second.PF << ", " << b.second.second.ST << ", "
+        << b.second.second.W << "]}";
     first = false;
   }
   rep << "]}";
@@ -1875,7 +1890,7 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   return rep.str();
 }
 
-// The tuning cache file: one line per entry, "TX TY VX PF ST <tune key>".
+// The tuning cache file: one line per entry, "TX TY VX PF ST W <tune key>".
 void Runtime::tune_load(const std::string& path) {
   std::ifstream f(path);
   if (!f) return;
@@ -1884,7 +1899,7 @@ void Runtime::tune_load(const std::string& path) {
     std::istringstream l(line);
     TileShape t;
     std::string key;
-    if (!(l >> t.TX >> t.TY >> t.VX >> t.PF >> t.ST)) continue;
+    if (!(l >> t.TX >> t.TY >> t.VX >> t.PF >> t.ST >> t.W)) continue;
     std::getline(l, key);
     if (!key.empty() && key[0] == ' ') key.erase(0, 1);
     if (!key.empty() && t.TX > 0 && t.VX > 0 && t.TY > 0) tuned_[key] = t;
@@ -1895,7 +1910,7 @@ void Runtime::tune_save(const std::string& path) const {
   std::ofstream f(path, std::ios::trunc);
   for (auto& e : tuned_)
     f << e.second.TX << " " << e.second.TY << " " << e.second.VX << " " << e.second.PF << " " << e.second.ST << " "
-      << e.first << "\n";
+      << e.second.W << " " << e.first << "\n";
 }
 
 }  // namespace mapx

@@ -196,7 +196,8 @@ def test_rand_matches_oracle(G, O, dt):
 
 def test_every_autotuner_candidate_gives_identical_bits(G, O):
     """Life and hillshade evaluated with each candidate tile shape of the autotuner (LDS-staged,
-    prefetching and register-window forms): identical results; Life equal to the oracle."""
+    prefetching and register-window forms, and the best shapes with registers capped for 6 and 8
+    wavefronts per SIMD): identical results; Life equal to the oracle."""
     rng = np.random.Generator(np.random.PCG64(34))
     s0 = (rng.random((520, 700)) < 0.3).astype(np.uint8)
     bs = (256, 128)
@@ -210,7 +211,7 @@ def test_every_autotuner_candidate_gives_identical_bits(G, O):
     hs0 = None
     try:
         for c in cands:
-            for key, val in zip(("tile_tx", "tile_ty", "tile_vx", "tile_pf", "tile_st"), c):
+            for key, val in zip(("tile_tx", "tile_ty", "tile_vx", "tile_pf", "tile_st", "tile_w"), c):
                 G.set_option(key, val)
             got = G.to_array(W.life(G, G.from_array(s0, bs), 6))
             assert_exact(got, life_ref, "life tile %s" % (c,))
@@ -219,7 +220,7 @@ def test_every_autotuner_candidate_gives_identical_bits(G, O):
                 hs0 = hs
             assert_exact(hs, hs0, "hillshade tile %s" % (c,))
     finally:
-        for key in ("tile_tx", "tile_ty", "tile_vx", "tile_pf", "tile_st"):
+        for key in ("tile_tx", "tile_ty", "tile_vx", "tile_pf", "tile_st", "tile_w"):
             G.set_option(key, 0)
 
 
