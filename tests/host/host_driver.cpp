@@ -251,6 +251,42 @@ void scenario_serving(std::mt19937& rng) {
   }
 }
 
+// serving while rasters spill: held DEMs past a small cap, requests reading them
+void scenario_serving_under_cap(std::mt19937& rng) {
+  std::vector<R> dems;
+  for (int i = 0; i < 6; i++) {
+    dems.push_back(slope(input(400, 300, 128, 128, rng)));  // computed rasters: spilled, not dropped
+    eval({dems.back()});
+  }
+  if (ma_setOption("hbm_cap", 4LL << 20) != 0) fail("setOption hbm_cap");
+  if (ma_serveOpen(3LL << 20, 4) != 0) fail("serveOpen");
+  long long submitted = 0, done = 0;
+  std::vector<std::vector<R>> keep;
+  for (int step = 0; step < 120; step++) {
+    if (submitted < 60) {
+      std::vector<R> t{shade(dems[rng() % dems.size()])};
+      std::vector<ma_node_t*> p{t[0].n};
+      if (ma_serveSubmit(p.data(), 1) < 0) fail("serveSubmit");
+      keep.push_back(t);
+      submitted++;
+    }
+    if (ma_serveStep(0) < 0) fail("serveStep");
+    long long ids[64];
+    double a[64], b[64], c[64];
+    const int n = ma_serveTake(ids, a, b, c, 64);
+    if (n < 0) fail("serveTake");
+    done += n;
+  }
+  if (ma_serveClose() != 0) fail("serveClose");
+  long long ids[64];
+  double a[64], b[64], c[64];
+  int n;
+  while ((n = ma_serveTake(ids, a, b, c, 64)) > 0) done += n;
+  if (done != submitted) fail("requests under the cap (not all completed)");
+  ma_setOption("hbm_cap", 0);
+  for (auto& t : keep) fetch(t[0]);
+}
+
 void scenario_churn(std::mt19937& rng) {
   R d = input(128, 128, 64, 64, rng);
   for (int i = 0; i < 3000; i++) {
@@ -275,6 +311,7 @@ int main() {
   ma_setOption("form", 1);
   scenario_cache_and_spill(rng);
   scenario_serving(rng);
+  scenario_serving_under_cap(rng);
   scenario_churn(rng);
   std::printf("host driver: ok %s\n", ma_runStats());
   return 0;
