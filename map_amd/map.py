@@ -178,13 +178,14 @@ def _node(ptr):
 
 class Raster:
     """Handle of one lazy DAG node (a raster, or a D0 value)."""
+    __slots__ = ('_node', '__weakref__')
 
     def __init__(self, node=None):
         if isinstance(node, Node):
             if not node:
                 _err(node)
             self._node = node
-            _lib.ma_increaseRef(self)
+            _fast.ma_increaseRef(node)
         elif node is None:
             self._node = Node(None)
         else:
@@ -192,8 +193,8 @@ class Raster:
 
     def __del__(self):
         try:
-            if _lib is not None and self._node:
-                _lib.ma_decreaseRef(self)
+            if _fast is not None and self._node:
+                _fast.ma_decreaseRef(self._node)
         except Exception:
             pass
 
@@ -330,7 +331,7 @@ def _scalar_fun(symbol):
 def _def_unary_fun(symbol, utype):
     def ufun(raster):
         if isinstance(raster, Raster):
-            return Raster(_lib.ma_unary(raster, utype))
+            return Raster(_fast.ma_unary(raster._node, utype))
         return _scalar_fun(symbol)(raster)
     ufun.__name__ = symbol
     globals()[symbol] = ufun
@@ -345,10 +346,10 @@ for _sym, _ut in zip(UnaryTypeSym, UnaryTypeVal):
 
 def _def_binary_op(symbol, btype):
     def biop(self, rhs):
-        return Raster(_lib.ma_binary(self, _const_to_raster(rhs), btype))
+        return Raster(_fast.ma_binary(self._node, _const_to_raster(rhs)._node, btype))
 
     def rbiop(self, lhs):
-        return Raster(_lib.ma_binary(_const_to_raster(lhs), self, btype))
+        return Raster(_fast.ma_binary(_const_to_raster(lhs)._node, self._node, btype))
     setattr(Raster, symbol, biop)
     setattr(Raster, symbol[:2] + 'r' + symbol[2:], rbiop)
 
@@ -495,7 +496,7 @@ def _mask(lst):
 
 def convolve(raster, lst):
     ptr, n, ds, keep = _mask(lst)
-    return Raster(_lib.ma_convolution(raster, ptr, n, ds))
+    return Raster(_fast.ma_convolution(raster._node, ptr, n, ds))
 
 
 def fsum(raster):
@@ -931,6 +932,17 @@ def _proto(name, args, res):
     f = getattr(_lib, name)
     f.argtypes = args
     f.restype = res
+
+
+# The node factories and reference counts the scripts call most, through a second
+# handle of the library whose argument types are the Node pointers themselves
+# (converted by ctypes without a Python call per argument).
+_fast = ct.CDLL(_LIBPATH)
+for _name, _args in (('ma_increaseRef', [Node]), ('ma_decreaseRef', [Node]),
+                     ('ma_binary', [Node, Node, ct.c_int]), ('ma_unary', [Node, ct.c_int]),
+                     ('ma_convolution', [Node, ct.POINTER(Variant), ct.c_int, Array])):
+    getattr(_fast, _name).argtypes = _args
+    getattr(_fast, _name).restype = None if _name.endswith('Ref') else Node
 
 
 _proto('ma_lastError', [], ct.c_char_p)
