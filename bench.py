@@ -172,7 +172,19 @@ def hillshade_zonal_bench(M, side=32768, block=1024, reps=3, cpu=True):
     rz = np.random.Generator(np.random.PCG64(3))
     zones = M.gen_zones((side, side), (block, block), rz.integers(0, side, 256), rz.integers(0, side, 256))
     cells = side * side
-    out = {}
+    # tile shapes are tuned per computation and block size: here on a 4096^2 raster of the
+    # same generators (256 zones) in the same blocks
+    ts = min(side, 4096)
+    rt = np.random.Generator(np.random.PCG64(2))
+    tdem = M.gen_dem((ts, ts), (block, block), W.dem_gaussians(rt, ts, ts), 2)
+    rzt = np.random.Generator(np.random.PCG64(3))
+    tzones = M.gen_zones((ts, ts), (block, block), rzt.integers(0, ts, 256), rzt.integers(0, ts, 256))
+    out = {"tuned_tiles": {}}
+    for name, fz in (("fused", 1), ("unfused", 0)):
+        M.set_option("focal_zonal", fz)
+        _progress("hillshade_zonal: autotuning the %s form" % name)
+        out["tuned_tiles"][name] = M.autotune(W.hillshade_zonal(M, tdem, tzones, 256), reps=3)["chosen"]
+    del tdem, tzones
     for name, fz in (("fused", 1), ("unfused", 0)):
         M.set_option("focal_zonal", fz)
         zs = W.hillshade_zonal(M, dem, zones, 256)
@@ -580,7 +592,7 @@ def hillshade_bench(M, side=4096, block=512, reps=5, cpu=True):
     from map_amd import workloads as W
     rng = np.random.Generator(np.random.PCG64(0))
     dem = M.gen_dem((side, side), (block, block), W.dem_gaussians(rng, side, side), 0)
-    out = {}
+    out = {"tuned_tiles": M.autotune(W.hillshade(M, dem), reps=3)["chosen"]}
     for name, form in (("batched", 1), ("plain", 0)):
         M.set_option("form", form)
         hs = W.hillshade(M, dem)

@@ -1821,49 +1821,65 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   const TileShape saved = opt.tile;
   const bool saved_graph = opt.graph, saved_prof = dev::profile_on();
   opt.graph = false;
-  // best average kernel time per tune key
-  std::map<std::string, std::pair<double, TileShape>> best;
+  // Each candidate is timed by its fastest launch (the least disturbed). Pass 1
+  // times every shape; pass 2 times again, side by side, the three fastest
+  // shapes of each computation and the fastest with its registers capped for 6
+  // or 8 wavefronts per SIMD (more work-groups per CU; a kernel that would spill
+  // loses), and the choice comes from pass 2 alone.
+  std::map<std::string, std::vector<std::pair<double, TileShape>>> times1;  // per tune key, pass 1
+  std::map<std::string, std::pair<double, TileShape>> best;                  // per tune key, pass 2
   std::ostringstream rep;
   rep << "{\"candidates\": [";
   bool first = true;
-  // second pass: the best shapes again with the registers capped for 6 or 8
-  // wavefronts per SIMD (more work-groups per CU; a kernel that would spill loses)
   std::vector<TileShape> pass(std::begin(cands), std::end(cands));
   try {
-   for (int stage = 0; stage < 2; stage++) {
-    if (stage == 1) {
-      pass.clear();
-      for (auto& b : best)
-        for (int w : {6, 8}) {
-          TileShape c = b.second.second;
-          c.W = w;
+    for (int stage = 0; stage < 2; stage++) {
+      if (stage == 1) {
+        pass.clear();
+        auto add = [&](const TileShape& c) {
           if (std::find(pass.begin(), pass.end(), c) == pass.end()) pass.push_back(c);
+        };
+        for (auto& kv : times1) {
+          auto v = kv.second;
+          std::stable_sort(v.begin(), v.end(), [](const std::pair<double, TileShape>& a,
+                                                  const std::pair<double, TileShape>& b) { return a.first < b.first; });
+          for (size_t i = 0; i < v.size() && i < 3; i++) add(v[i].second);
+          for (int w : {6, 8}) {
+            TileShape c = v[0].second;
+            c.W = w;
+            add(c);
+          }
         }
-    }
-    for (const TileShape& c : pass) {
-      opt.tile = c;
-      for (Node* t : targets) drop(t);
-      evaluate(targets);  // compile and warm
-      dev::profile_set(true);
-      dev::profile_take();
-      for (int r = 0; r < reps; r++) {
+      }
+      for (const TileShape& c : pass) {
+        opt.tile = c;
         for (Node* t : targets) drop(t);
-        evaluate(targets);
-      }
-      auto times = dev::profile_take();
-      dev::profile_set(false);
-      for (auto& k : times) {
-        auto kg = kernel_group_.find(k.name);
-        if (kg == kernel_group_.end()) continue;
-        const double avg = k.total_ms / k.count;
-        auto b = best.find(kg->second);
-        if (b == best.end() || avg < b->second.first) best[kg->second] = {avg, c};
-        rep << (first ? "" : ", ") << "{\"tile\": [" << c.TX << ", " << c.TY << ", " << c.VX << ", " << c.PF << ", " << c.ST << ", " << c.W << "], \"kernel\": \""
-            << k.name << "\", \"avg_ms\": " << avg << "}";
-        first = false;
+        evaluate(targets);  // compile and warm
+        dev::profile_set(true);
+        dev::profile_take();
+        for (int r = 0; r < reps; r++) {
+          for (Node* t : targets) drop(t);
+          evaluate(targets);
+        }
+        auto times = dev::profile_take();
+        dev::profile_set(false);
+        for (auto& k : times) {
+          auto kg = kernel_group_.find(k.name);
+          if (kg == kernel_group_.end()) continue;
+          const double t = k.min_ms;
+          if (stage == 0) {
+            times1[kg->second].push_back({t, c});
+          } else {
+            auto b = best.find(kg->second);
+            if (b == best.end() || t < b->second.first) best[kg->second] = {t, c};
+          }
+          rep << (first ? "" : ", ") << "{\"pass\": " << stage + 1 << ", \"tile\": [" << c.TX << ", " << c.TY << ", " << c.VX
+              << ", " << c.PF << ", " << c.ST << ", " << c.W << "], \"kernel\": \"" << k.name << "\", \"min_ms\": " << t
+              << ", \"avg_ms\": " << k.total_ms / k.count << "}";
+          first = false;
+        }
       }
     }
-   }
   } catch (...) {
     opt.tile = saved;
     opt.graph = saved_graph;
@@ -1877,7 +1893,7 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   first = true;
   for (auto& b : best) {
     tuned_[b.first] = b.second.second;
-    rep << (first ? "" : ", ") << "{\"avg_ms\": " << b.second.first << ", \"tile\": [" << b.second.second.TX << ", "
+    rep << (first ? "" : ", ") << "{\"min_ms\": " << b.second.first << ", \"tile\": [" << b.second.second.TX << ", "
         << b.second.second.TY << ", " << b.second.second.VX << ", " << b.second.second.PF << ", " << b.second.second.ST << ", "
         << b.second.second.W << "]}";
     first = false;
