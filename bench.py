@@ -172,19 +172,13 @@ def hillshade_zonal_bench(M, side=32768, block=1024, reps=3, cpu=True):
     rz = np.random.Generator(np.random.PCG64(3))
     zones = M.gen_zones((side, side), (block, block), rz.integers(0, side, 256), rz.integers(0, side, 256))
     cells = side * side
-    # tile shapes are tuned per computation and block size: here on a 4096^2 raster of the
-    # same generators (256 zones) in the same blocks
-    ts = min(side, 4096)
-    rt = np.random.Generator(np.random.PCG64(2))
-    tdem = M.gen_dem((ts, ts), (block, block), W.dem_gaussians(rt, ts, ts), 2)
-    rzt = np.random.Generator(np.random.PCG64(3))
-    tzones = M.gen_zones((ts, ts), (block, block), rzt.integers(0, ts, 256), rzt.integers(0, ts, 256))
+    # tile shapes tuned on the workload itself (a smaller raster of the same script
+    # ranks the shapes differently: fewer blocks to spread over the CUs)
     out = {"tuned_tiles": {}}
     for name, fz in (("fused", 1), ("unfused", 0)):
         M.set_option("focal_zonal", fz)
         _progress("hillshade_zonal: autotuning the %s form" % name)
-        out["tuned_tiles"][name] = M.autotune(W.hillshade_zonal(M, tdem, tzones, 256), reps=3)["chosen"]
-    del tdem, tzones
+        out["tuned_tiles"][name] = M.autotune(W.hillshade_zonal(M, dem, zones, 256), reps=2)["chosen"]
     for name, fz in (("fused", 1), ("unfused", 0)):
         M.set_option("focal_zonal", fz)
         zs = W.hillshade_zonal(M, dem, zones, 256)
