@@ -236,10 +236,12 @@ def batched_tiles_bench(M, ntiles=2048, block=1024, seed=4, forms=("batched", "p
     M.set_option("focal_zonal", 1)
     res = {}
     try:
-        # tile shapes tuned on the first 64 tiles (keys: computation and block size)
-        sample = [W.batch_tile_sums(M, d, z, w) for d, z, w in tiles[:64]]
-        _progress("batched tiles: autotuning on 64 tiles")
-        tuned = M.autotune(*sample, reps=1)["chosen"]
+        # tile shapes tuned on the whole population, batched as it is timed (keys:
+        # computation and block size; a sample of the first tiles ranks the shapes differently)
+        M.set_option("form", 1)
+        sample = [W.batch_tile_sums(M, d, z, w) for d, z, w in tiles]
+        _progress("batched tiles: autotuning on the %d tiles" % len(tiles))
+        tuned = M.autotune(*sample, reps=2)["chosen"]
         del sample
         for name, form in (("batched", 1), ("plain", 0)):
             if name not in forms:

@@ -1821,7 +1821,8 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   const TileShape saved = opt.tile;
   const bool saved_graph = opt.graph, saved_prof = dev::profile_on();
   opt.graph = false;
-  // Each candidate is timed by its fastest launch (the least disturbed). Pass 1
+  // Each candidate is timed by its kernel's time in one evaluation (the sum of
+  // its launches), the least over the repetitions (the least disturbed). Pass 1
   // times every shape; pass 2 times again, side by side, the three fastest
   // shapes of each computation and the fastest with its registers capped for 6
   // or 8 wavefronts per SIMD (more work-groups per CU; a kernel that would spill
@@ -1857,16 +1858,25 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
         evaluate(targets);  // compile and warm
         dev::profile_set(true);
         dev::profile_take();
+        // per kernel: its time in one evaluation (all its launches), the least over the repetitions
+        std::map<std::string, std::pair<double, double>> per;  // name -> (least evaluation total, sum)
+        std::map<std::string, long> launches;
         for (int r = 0; r < reps; r++) {
           for (Node* t : targets) drop(t);
           evaluate(targets);
+          for (auto& k : dev::profile_take()) {
+            auto it = per.find(k.name);
+            if (it == per.end()) per[k.name] = {k.total_ms, k.total_ms};
+            else { it->second.first = std::min(it->second.first, k.total_ms); it->second.second += k.total_ms; }
+            launches[k.name] += k.count;
+          }
         }
-        auto times = dev::profile_take();
         dev::profile_set(false);
-        for (auto& k : times) {
+        for (auto& kv : per) {
+          struct { std::string name; long count; double total_ms; } k{kv.first, launches[kv.first], kv.second.second};
           auto kg = kernel_group_.find(k.name);
           if (kg == kernel_group_.end()) continue;
-          const double t = k.min_ms;
+          const double t = kv.second.first;
           if (stage == 0) {
             times1[kg->second].push_back({t, c});
           } else {
@@ -1874,8 +1884,8 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
             if (b == best.end() || t < b->second.first) best[kg->second] = {t, c};
           }
           rep << (first ? "" : ", ") << "{\"pass\": " << stage + 1 << ", \"tile\": [" << c.TX << ", " << c.TY << ", " << c.VX
-              << ", " << c.PF << ", " << c.ST << ", " << c.W << "], \"kernel\": \"" << k.name << "\", \"min_ms\": " << t
-              << ", \"avg_ms\": " << k.total_ms / k.count << "}";
+              << ", " << c.PF << ", " << c.ST << ", " << c.W << "], \"kernel\": \"" << k.name << "\", \"eval_ms\": " << t
+              << ", \"avg_ms\": " << k.total_ms / std::max(1L, k.count) << "}";
           first = false;
         }
       }
@@ -1893,7 +1903,7 @@ std::string Runtime::autotune(const std::vector<Node*>& targets, int reps) {
   first = true;
   for (auto& b : best) {
     tuned_[b.first] = b.second.second;
-    rep << (first ? "" : ", ") << "{\"min_ms\": " << b.second.first << ", \"tile\": [" << b.second.second.TX << ", "
+    rep << (first ? "" : ", ") << "{\"eval_ms\": " << b.second.first << ", \"tile\": [" << b.second.second.TX << ", "
         << b.second.second.TY << ", " << b.second.second.VX << ", " << b.second.second.PF << ", " << b.second.second.ST << ", "
         << b.second.second.W << "]}";
     first = false;
