@@ -93,6 +93,132 @@ std::string binary_stmt(const Node* n, const std::string& l, const std::string& 
   return std::string("(") + CT(dt) + ")" + fn + "((double)" + l + ", (double)" + r + ")";
 }
 
+// ---- byte-lane form ---------------------------------------------------------------
+//
+// Exact value range of a node of a focal group over one U8 raster `s`, for staged
+// values in [0, R]. Uniform nodes (D0 inputs, constants) are taken in [0, 127]; the
+// kernel checks its D0 arguments against that before it uses the lanes.
+struct LaneRange {
+  int lo = 0, hi = 0;
+  bool uni = false;
+};
+
+bool lane_dt_ok(int dt, int lo, int hi) {
+  if (is_float_dt(dt)) return false;
+  if (dt == S8) return lo >= -128 && hi <= 127;
+  return lo >= 0;  // every other integer type holds [0, 255]
+}
+
+// Ranges of the nodes of `all` (kernel order), or false when some node has no exact
+// byte-lane statement: 0/1-weighted convolutions and neighbours of `s`, casts, sums,
+// differences that stay non-negative, products with a 0/1 operand, comparisons of
+// operands in [0, 127], logical and bitwise and/or/xor, conditionals on a 0/1
+// condition, with every value in [0, 255] (so the C conversions are identities).
+bool lane_ranges(const std::vector<Node*>& all, const std::set<Node*>& in_set, const Node* s, int R,
+                 std::map<const Node*, LaneRange>& rg) {
+  rg.clear();
+  for (Node* n : all) {
+    LaneRange r;
+    if (in_set.count(n)) {
+      if (is_float_dt(n->meta.dt)) return false;
+      if (n->meta.nd == D2) {
+        if (n != s) return false;
+        r.hi = R;
+      } else {
+        r.hi = 127;
+        r.uni = true;
+      }
+      rg[n] = r;
+      continue;
+    }
+    if (is_float_dt(n->meta.dt)) return false;
+    for (Node* p : n->prev)
+      if (!rg.count(p)) return false;
+    auto arg = [&](int i) { return rg.at(n->prev[i]); };
+    switch (n->kind) {
+      case Kind::Constant: {
+        if (is_float_dt(n->cnst.dtype)) return false;
+        const long long v = variant_as<long long>(n->cnst);
+        if (v < 0 || v > 127) return false;
+        r.lo = r.hi = (int)v;
+        r.uni = true;
+        break;
+      }
+      case Kind::Neighbor:
+        if (n->prev[0] != s) return false;
+        r.hi = R;
+        break;
+      case Kind::Convolution: {
+        if (n->prev[0] != s || is_float_dt(n->mask.dtype())) return false;
+        int w = 0;
+        for (int i1 = 0; i1 < n->mask.ds.arr[1]; i1++)
+          for (int i0 = 0; i0 < n->mask.ds.arr[0]; i0++) {
+            const long long m = variant_as<long long>(n->mask.at(i0, i1));
+            if (m < 0 || m > 1) return false;
+            w += (int)m;
+          }
+        r.hi = R * w;
+        break;
+      }
+      case Kind::Cast:
+        r = arg(0);
+        break;
+      case Kind::Unary: {
+        const LaneRange a = arg(0);
+        if (n->op == POS) r = a;
+        else if (n->op == NOT && !a.uni && a.lo >= 0 && a.hi <= 1) r.hi = 1;
+        else return false;
+        break;
+      }
+      case Kind::Binary: {
+        const LaneRange a = arg(0), b = arg(1);
+        if (a.uni && b.uni) return false;
+        if (is_float_dt(n->prev[0]->meta.dt) || is_float_dt(n->prev[1]->meta.dt)) return false;
+        const bool a01 = !a.uni && a.lo >= 0 && a.hi <= 1, b01 = !b.uni && b.lo >= 0 && b.hi <= 1;
+        switch (n->op) {
+          case ADD: r.lo = a.lo + b.lo; r.hi = a.hi + b.hi; break;
+          case SUB:
+            if (a.uni || b.uni) return false;
+            r.lo = a.lo - b.hi; r.hi = a.hi - b.lo;
+            break;
+          case MUL:
+            if (a.uni || b.uni || !(a01 || b01)) return false;
+            r.lo = a.lo * b.lo; r.hi = a.hi * b.hi;
+            break;
+          case EQ: case NE: case LT: case GT: case LE: case GE:
+            if (a.lo < 0 || b.lo < 0 || a.hi > 127 || b.hi > 127) return false;
+            r.hi = 1;
+            break;
+          case AND: case OR:
+            if (!a01 || !b01) return false;
+            r.hi = 1;
+            break;
+          case bAND: r.hi = std::min(a.hi, b.hi); break;
+          case bOR: case bXOR: {
+            int p = 1;
+            while (p <= std::max(a.hi, b.hi)) p <<= 1;
+            r.hi = p - 1;
+            break;
+          }
+          default: return false;
+        }
+        break;
+      }
+      case Kind::Conditional: {
+        const LaneRange c = arg(0), a = arg(1), b = arg(2);
+        if (c.uni || c.lo < 0 || c.hi > 1) return false;
+        r.lo = std::min(a.lo, b.lo);
+        r.hi = std::max(a.hi, b.hi);
+        break;
+      }
+      default: return false;
+    }
+    if (r.lo < 0 || r.hi > 255 || !lane_dt_ok(n->meta.dt, r.lo, r.hi)) return false;
+    rg[n] = r;
+  }
+  return true;
+}
+
 }  // namespace
 
 Variant quantize_literal(const Variant& v) {
@@ -336,6 +462,24 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     }
     ks.stream = ok;
   }
+  // byte-lane form: a staged focal group over one U8 input raster with one 1-byte output,
+  // every node exact in byte lanes for staged values in [0, R]. R = 255 needs no check of
+  // the data; otherwise R = 1 (0/1 rasters, where products of 0/1 values are ANDs). A
+  // group exact for some R is exact for every smaller one, so there is no other choice.
+  std::map<const Node*, LaneRange> lane_rg;
+  Node* lane_s = staged.size() == 1 ? staged[0] : nullptr;
+  Node* lane_out = d2_outs.size() == 1 ? d2_outs[0] : nullptr;
+  if (ks.form == Form::Focal && !ks.stream && zonal.empty() && lane_s && lane_out && in_set.count(staged[0]) &&
+      lane_s->meta.nd == D2 && (lane_s->meta.dt == U8 || lane_s->meta.dt == B8) && VX % 4 == 0 &&
+      !in_set.count(lane_out) && sizeof_dt(lane_out->meta.dt) == 1 &&
+      std::none_of(cell_inputs.begin(), cell_inputs.end(), [](Node* n) { return n->meta.nd == D2; })) {
+    for (int R : {255, 1})
+      if (lane_ranges(all, in_set, lane_s, R, lane_rg)) {
+        ks.lane_r = (unsigned)R;
+        break;
+      }
+    if (!ks.lane_r) lane_rg.clear();
+  }
   s << "#include \"mapx_dev.hiph\"\n";
   // the module for rasters with an edge block narrower than the halo is the
   // same source compiled with MX_NARROW 1 (narrow_source)
@@ -353,6 +497,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   s << "  };\n";
   s << "  struct Smem {\n";
   for (Node* st : staged) s << "    alignas(16) " << CT(st->meta.dt) << " sh" << LN(st) << "[" << SW * HH << "];\n";
+  if (ks.lane_r) s << "    int lane_bad;  // the last tile with a staged value above LANE_R\n";
   if (staged.empty()) s << "    int unused;\n";
   s << "  };\n";
   // per-thread register window of every staged node
@@ -557,8 +702,9 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
   for (Node* n : raw_in)
     s << "    mx::ldpack<" << CT(n->meta.dt) << ", VX>(a.i" << LN(n) << ", rs, x0v, r.b" << LN(n) << ");\n";
   s << "    (void)a; (void)rs; (void)x0v; (void)r;\n  }\n";
+  // both return, for a byte-lane group, the OR of the staged values (else 0)
   auto pre_body = [&](const char* fname, bool put) {
-    s << "  template <int VX" << (put ? "" : ", bool NW") << "> __device__ static __forceinline__ void " << fname
+    s << "  template <int VX" << (put ? "" : ", bool NW") << "> __device__ static __forceinline__ mx::u32 " << fname
       << "(const Args& a, const mx::Blk& k, " << (put ? "const Raw<VX>& r, " : "") << "int x0v, int y, Smem& sm, int si) {\n";
     for (Node* n : raw_in) {
       s << "    " << CT(n->meta.dt) << " b" << LN(n) << "[VX];\n";
@@ -577,7 +723,12 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
     for (Node* st : staged) s << "      t" << LN(st) << "[v] = " << V(st) << ";\n";
     s << "      (void)x;\n    }\n";
     for (Node* st : staged) s << "    mx::stvec<" << CT(st->meta.dt) << ", VX>(&sm.sh" << LN(st) << "[si], t" << LN(st) << ");\n";
-    s << "    (void)a; (void)k; (void)y; (void)sm; (void)si;\n  }\n";
+    s << "    (void)a; (void)k; (void)y; (void)sm; (void)si;\n";
+    if (!ks.lane_r) s << "    return 0u;\n";
+    else if (put) s << "    return mx::word_or(r.b" << LN(lane_s) << ".w);\n";
+    else s << "    mx::u32 m = 0u;\n#pragma unroll\n    for (int v = 0; v < VX; v++) m |= (mx::u32)t" << LN(lane_s)
+           << "[v];\n    return m;\n";
+    s << "  }\n";
   };
   pre_body("pre_vec", false);
   pre_body("put_row", true);
@@ -600,6 +751,97 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
       << "      else for (int v = 0; v < nv; v++) mx::gst(a.o" << L << " + off + v, t[v]);\n    }\n";
   }
   s << "    (void)a; (void)off; (void)nv; (void)o;\n  }\n";
+  // byte-lane members: post_lanes() computes the thread's VX cells four to a dword from
+  // its window rows read from LDS as dwords; store_lanes() writes them
+  s << "  static constexpr unsigned LANE_R = " << ks.lane_r << "u;\n";
+  if (ks.lane_r) {
+    const int WD = WW / 4;
+    auto X = [&](const Node* p) { return std::string(lane_rg.at(p).uni ? "U" : "L") + LN(p); };
+    auto tap = [&](int dx, int dy) {
+      // lane_at<dx> reads dwords c + floor(dx / 4) and, off a dword boundary, the next one
+      for (int j = 0; j < VX / 4; j++) {
+        const int b = PAD + 4 * j + dx;
+        if (b < 0 || b + 3 >= WW) throw std::runtime_error("codegen: byte-lane tap outside the window");
+      }
+      return "mx::lane_at<" + std::to_string(dx) + ">(w[" + std::to_string(ks.H1 + dy) + "], c)";
+    };
+    s << "  __device__ static __forceinline__ bool lane_args_ok(const Args& a) {\n    (void)a;\n    return true";
+    for (Node* n : all)
+      if (in_set.count(n) && n->meta.nd == D0)
+        s << " && (long long)a.s" << LN(n) << " >= 0 && (long long)a.s" << LN(n) << " <= 127";
+    s << ";\n  }\n";
+    s << "  template <int VX> __device__ static __forceinline__ void post_lanes(const Args& a, const Smem& sm, int base, mx::u32 (&o)[VX / 4]) {\n";
+    s << "    static_assert(VX == " << VX << ", \"byte-lane window\");\n";
+    s << "    mx::u32 w[" << WR << "][" << WD << "];\n";
+    s << "#pragma unroll\n    for (int r = 0; r < " << WR << "; r++)\n#pragma unroll\n      for (int c = 0; c < " << WW / VX
+      << "; c++) {\n        const mx::Vec<mx::u32, VX / 4> t = *(const mx::Vec<mx::u32, VX / 4>*)&sm.sh" << LN(lane_s)
+      << "[base + r * SW + c * VX];\n#pragma unroll\n        for (int e = 0; e < VX / 4; e++) w[r][c * (VX / 4) + e] = t.v[e];\n      }\n";
+    // uniform values, replicated into every lane
+    for (Node* n : all) {
+      if (!lane_rg.at(n).uni) continue;
+      if (in_set.count(n)) s << "    const mx::u32 U" << LN(n) << " = mx::lane_rep((int)a.s" << LN(n) << ");\n";
+      else if (n->kind == Kind::Constant) s << "    const mx::u32 U" << LN(n) << " = mx::lane_rep(" << lane_rg.at(n).lo << ");\n";
+      else s << "    const mx::u32 U" << LN(n) << " = " << X(n->prev[0]) << ";\n";  // a cast
+    }
+    s << "#pragma unroll\n    for (int j = 0; j < VX / 4; j++) {\n      const int c = " << PAD / 4 << " + j;\n";
+    for (Node* n : all) {
+      const LaneRange& r = lane_rg.at(n);
+      if (r.uni) continue;
+      const std::string L = "      const mx::u32 L" + LN(n) + " = ";
+      if (n == lane_s) { s << L << tap(0, 0) << ";\n"; continue; }
+      switch (n->kind) {
+        case Kind::Neighbor: s << L << tap(n->coord.arr[0], n->coord.arr[1]) << ";\n"; break;
+        case Kind::Convolution: {
+          s << L << "0u";
+          const int h0 = n->halo(0), h1 = n->halo(1);
+          for (int i1 = -h1; i1 <= h1; i1++)
+            for (int i0 = -h0; i0 <= h0; i0++)
+              if (variant_as<long long>(n->mask.at(i0 + h0, i1 + h1)) != 0) s << " + " << tap(i0, i1);
+          s << ";\n";
+          break;
+        }
+        case Kind::Cast: s << L << X(n->prev[0]) << ";\n"; break;
+        case Kind::Unary: s << L << X(n->prev[0]) << (n->op == NOT ? " ^ 0x01010101u" : "") << ";\n"; break;
+        case Kind::Conditional:
+          s << L << "mx::lane_pick(" << X(n->prev[0]) << ", " << X(n->prev[1]) << ", " << X(n->prev[2]) << ");\n";
+          break;
+        case Kind::Binary: {
+          const Node *pa = n->prev[0], *pb = n->prev[1];
+          const std::string A = X(pa), B = X(pb);
+          const LaneRange &ra = lane_rg.at(pa), &rb = lane_rg.at(pb);
+          const bool a01 = !ra.uni && ra.hi <= 1, b01 = !rb.uni && rb.hi <= 1;
+          switch (n->op) {
+            case ADD: s << L << A << " + " << B << ";\n"; break;
+            case SUB: s << L << A << " - " << B << ";\n"; break;
+            case MUL:
+              if (a01 && b01) s << L << A << " & " << B << ";\n";
+              else if (a01) s << L << "mx::lane_sel(" << A << ", " << B << ");\n";
+              else s << L << "mx::lane_sel(" << B << ", " << A << ");\n";
+              break;
+            case EQ: s << L << "mx::lane_eq(" << A << ", " << B << ");\n"; break;
+            case NE: s << L << "mx::lane_ne(" << A << ", " << B << ");\n"; break;
+            case LT: s << L << "mx::lane_lt(" << A << ", " << B << ");\n"; break;
+            case GE: s << L << "mx::lane_ge(" << A << ", " << B << ");\n"; break;
+            case GT: s << L << "mx::lane_lt(" << B << ", " << A << ");\n"; break;
+            case LE: s << L << "mx::lane_ge(" << B << ", " << A << ");\n"; break;
+            case AND: case bAND: s << L << A << " & " << B << ";\n"; break;
+            case OR: case bOR: s << L << A << " | " << B << ";\n"; break;
+            case bXOR: s << L << A << " ^ " << B << ";\n"; break;
+            default: throw std::runtime_error("codegen: byte-lane operator");
+          }
+          break;
+        }
+        default:
+          if (in_set.count(n)) continue;  // the staged input is read through taps
+          throw std::runtime_error("codegen: byte-lane node kind");
+      }
+    }
+    s << "      o[j] = L" << LN(lane_out) << ";\n    }\n    (void)a;\n  }\n";
+    s << "  template <int VX> __device__ static __forceinline__ void store_lanes(const Args& a, mx::i64 off, int nv, const mx::u32 (&o)[VX / 4]) {\n";
+    s << "    unsigned char* p = (unsigned char*)(a.o" << LN(lane_out) << " + off);\n";
+    s << "    if (nv == VX) mx::gstw<VX / 4>(p, o);\n";
+    s << "    else\n      for (int v = 0; v < nv; v++) p[v] = (unsigned char)(o[v >> 2] >> (8 * (v & 3)));\n  }\n";
+  }
   // load_win(): the thread's window rows from LDS with aligned vector reads
   s << "  template <int VX> __device__ static __forceinline__ void load_win(const Smem& sm, int base, Win& W) {\n";
   for (Node* st : staged) {

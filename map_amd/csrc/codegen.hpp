@@ -51,6 +51,7 @@ struct KernelSpec {
   std::string source;
   Form form = Form::Local;
   bool stream = false;  // focal group run by stream_tile (register row window) rather than the LDS tile
+  unsigned lane_r = 0;  // byte-lane form for staged values in [0, lane_r] (0: none)
   TileShape tile;
   int H0 = 0, H1 = 0;
   std::vector<Param> params;

@@ -1780,7 +1780,7 @@ std::string Runtime::plan(const std::vector<Node*>& targets_in) {
           continue;
         }
         j << ", \"kernel\": \"" << ks.name << "\", \"form\": " << (int)ks.form << ", \"halo\": [" << ks.H0 << ", " << ks.H1
-          << "], \"stream\": " << (ks.stream ? 1 : 0) << ", \"tile\": [" << ks.tile.TX << ", " << ks.tile.TY << ", " << ks.tile.VX << ", " << ks.tile.PF
+          << "], \"stream\": " << (ks.stream ? 1 : 0) << ", \"lane_r\": " << ks.lane_r << ", \"tile\": [" << ks.tile.TX << ", " << ks.tile.TY << ", " << ks.tile.VX << ", " << ks.tile.PF
           << "], \"tune_key\": \"" << esc(tune_key(*g)) << "\", \"shape_key\": \""
           << esc(kernel_shape_key(*g, kouts, tile_for(*g), opt.libm)) << "\", \"source\": \"" << esc(ks.source) << "\"";
         // floating mask weights as the kernel receives them (quantised like printed literals)

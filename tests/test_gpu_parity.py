@@ -112,6 +112,49 @@ def test_life_forms_identical(G, O):
         assert_exact(o, ref, "life forms")
 
 
+@pytest.mark.parametrize("case", ["01", "mixed", "u8", "narrow"])
+def test_byte_lane_groups_match_oracle(G, O, case):
+    """Groups with a byte-lane form (tests/test_byte_lanes.py) against the oracle on 0/1
+    rasters (every tile in byte lanes), on 0/1 rasters with scattered larger values (the
+    tiles holding one, through their halo too, take the per-cell form), on full-range U8
+    data (the per-cell form, except the group exact for any byte) and on a raster whose
+    edge blocks are narrower than the 7x7 halo. Right-edge vectors are partial (300 % 8)."""
+    from tests.test_byte_lanes import lane_scripts
+    print("case byte lanes", case, flush=True)
+    rng = np.random.Generator(np.random.PCG64(11))
+    w, h, bs = (258, 129, (128, 128)) if case == "narrow" else (300, 200, (128, 64))
+    a = (rng.random((h, w)) < 0.35).astype(np.uint8)
+    if case == "mixed":
+        idx = rng.integers(0, a.size, 40)
+        a.flat[idx] = rng.integers(2, 256, 40)
+    if case == "u8":
+        a = rng.integers(0, 256, (h, w)).astype(np.uint8)
+    gs, os_ = lane_scripts(G, G.from_array(a, bs)), lane_scripts(O, O.from_array(a, bs))
+    for name in gs:
+        assert_exact(G.to_array(gs[name][0]), os_[name][0].val, name)
+
+
+def test_life_byte_lanes_with_larger_values_and_constants(G, O):
+    """Life over 12 generations from 0/1 cells with a few larger values (they persist where
+    a cell has two neighbours, so tiles move between the two forms from one generation to
+    the next), and a rule whose constants lie outside the lanes' [0, 127] (the kernel's
+    argument check sends every tile to the per-cell form)."""
+    rng = np.random.Generator(np.random.PCG64(12))
+    a = (rng.random((384, 520)) < 0.3).astype(np.uint8)
+    a[100:108, 200:260] = 5
+    a[300, 17] = 200
+    bs = (128, 128)
+    got = G.to_array(scripts.life(G, G.from_array(a, bs), 12))
+    ref = scripts.life(O, O.from_array(a, bs), 12).val
+    assert_exact(got, ref, "life with larger values")
+
+    def rule(M, u):
+        nbh = M.convolve(u, scripts.LIFE_MASK)
+        return (nbh == 300) + (nbh == -1) * u + (nbh == 3)
+    b = (rng.random((200, 300)) < 0.3).astype(np.uint8)
+    assert_exact(G.to_array(rule(G, G.from_array(b, bs))), rule(O, O.from_array(b, bs)).val, "constants out of range")
+
+
 def test_local_integer_ops(G, O):
     rng = np.random.Generator(np.random.PCG64(3))
     a = rng.integers(0, 256, (100, 130)).astype(np.uint8)
