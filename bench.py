@@ -639,6 +639,8 @@ def main():
                     help="TX,TY,VX[,PF[,ST[,W]]] tile shape of the Life kernel (implies --tune 0)")
     ap.add_argument("--chunk-tiles", type=int, default=0,
                     help="tiles per chunk of the persistent kernels (0: the runtime's choice)")
+    ap.add_argument("--queue", type=int, default=-1, choices=[-1, 0, 1, 2],
+                    help="chunk split: 0 device queue for batched launches only (the default), 1 always, 2 never")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the Life headline (skip hillshade, hillshade+zonal, batched tiles, serving)")
     ap.add_argument("--only", choices=["hillshade", "hillshade_zonal", "batched_tiles", "serve", "radial", "spread",
@@ -657,6 +659,8 @@ def main():
     M.setupDevices("", M.DEV_GPU, "")
     if args.chunk_tiles:
         M.set_option("chunk_tiles", args.chunk_tiles)
+    if args.queue >= 0:
+        M.set_option("queue", args.queue)
     if args.only:
         fn = {"hillshade": hillshade_bench, "hillshade_zonal": hillshade_zonal_bench,
               "batched_tiles": batched_tiles_bench, "serve": serve_bench, "radial": radial_bench,
