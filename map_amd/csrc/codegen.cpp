@@ -474,7 +474,7 @@ KernelSpec generate_kernel(const Group& g, const std::vector<Node*>& d2_outs, Ti
       !in_set.count(lane_out) && sizeof_dt(lane_out->meta.dt) == 1 &&
       std::none_of(cell_inputs.begin(), cell_inputs.end(), [](Node* n) { return n->meta.nd == D2; })) {
     for (int R : {255, 1})
-      if (lane_ranges(all, in_set, lane_s, R, lane_rg)) {
+      if (lane_ranges(all, in_set, lane_s, R, lane_rg) && !lane_rg.at(lane_out).uni) {  // the output is per cell
         ks.lane_r = (unsigned)R;
         break;
       }
